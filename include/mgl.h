@@ -1,0 +1,207 @@
+/* mgl.h - C ABI of the MI355X Louvain engine (libmgl.so).
+ *
+ * The reference (NikodemGapski/MG-alphaGCD) has no plugin or FFI: its Louvain
+ * path sits behind the MG_GCD command line (src/main.cu:15-135, flags
+ * :20-26) and the in-process call
+ *     X::run(HostGraph*, GpuGraph*, double threshold, int max_iter, int max_phases)
+ * (include/louvain/louvain.cuh:11, louvain_gl.cuh:11, louvain_baseline.cuh:11)
+ * selected by -version (src/main.cu:118-131). Its internal seams are
+ * BIN::bin_create (include/bin/BIN.cuh:9-25), the binned move dispatcher
+ * calculate_eicj_and_move_vertex_bin (src/louvain/louvain.cu:1032), the weight
+ * update (:1299-1396), compute_modularity (:105-238) and
+ * coarsen_graph_mg::coarsen_graph (include/coarsen_graph/coarsen_graph_mg.cuh:9).
+ * Each entry point below names the reference interface it replaces.
+ *
+ * Conventions: plain pointers and sizes, no exceptions, no exit(); every call
+ * returns an mgl_status and the context keeps the last error string. One
+ * context = one device + one stream; a context is not thread safe, distinct
+ * contexts may be used from distinct threads. Inputs are borrowed for the
+ * duration of a call; results are owned by the library until freed.
+ */
+#ifndef MGL_H
+#define MGL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MGL_OK = 0,
+  MGL_INVALID_ARG = 1,
+  MGL_INVALID_GRAPH = 2,
+  MGL_OOM = 3,
+  MGL_DEVICE_ERROR = 4,
+  MGL_UNSUPPORTED = 5
+} mgl_status;
+
+/* Precision of the sums (weights, community totals, modularity). */
+enum {
+  MGL_PREC_AUTO = 0,     /* EXACT when the weights allow it, else FP64 */
+  MGL_PREC_EXACT = 1,    /* integer / dyadic fixed-point sums, bit-exact, order independent */
+  MGL_PREC_FP64 = 2,     /* double weights and sums (reference weight_t = double) */
+  MGL_PREC_FP32_HASH = 3 /* float edge weights and hash slots; FP64 for 2m, a_c and Q */
+};
+
+/* Tie rule between candidate communities of equal gain. */
+enum {
+  MGL_TIE_REFERENCE = 0, /* smallest id on up passes, largest on down passes (louvain.cu:544-554) */
+  MGL_TIE_SMALLEST = 1   /* always the smallest id */
+};
+
+/* Host CSR. Directed entries; an undirected edge appears in both rows. */
+typedef struct {
+  int64_t n;              /* vertices */
+  int64_t nnz;            /* directed CSR entries = offsets[n] */
+  const int64_t* offsets; /* [n+1], offsets[0] = 0, non-decreasing */
+  const int32_t* cols;    /* [nnz], 0 <= col < n */
+  const float* w;         /* [nnz] or NULL for unit weights */
+} mgl_csr;
+
+/* Options; MG_GCD flags -threshold, -max_phases, -max_iter, -version
+ * (src/main.cu:20-26) plus build-only switches. */
+typedef struct {
+  int version;         /* 0 = plain atomic, 1 = global bi-probing, 2 = local bi-probing (default) */
+  double threshold;    /* 1e-6 */
+  int max_phases;      /* 1000 */
+  int max_iter;        /* <= 0: unlimited (the reference ignores -max_iter) */
+  int precision;       /* MGL_PREC_* */
+  int tie_rule;        /* MGL_TIE_* */
+  int fuse_modularity; /* 1 (default): sum_in of a proposal comes from the next move pass */
+  int keep_levels;     /* 1: keep per-level assignments in the result */
+  int keep_graphs;     /* 1: keep per-level coarse CSRs in the result */
+  int validate;        /* 1 (default): check offsets, column range and symmetry */
+  const char* tuning_file; /* bin thresholds written by mgl_autotune, or NULL for built-in defaults */
+} mgl_opts;
+
+typedef struct mgl_ctx mgl_ctx;
+typedef struct mgl_result mgl_result;
+typedef struct mgl_session mgl_session;
+
+void mgl_opts_default(mgl_opts* o);
+
+/* Replaces the MPI/NVSHMEM bootstrap and per-node device selection
+ * (src/main.cu:30-109): one context per device. */
+mgl_status mgl_create(mgl_ctx** ctx, int device);
+void mgl_destroy(mgl_ctx* ctx);
+const char* mgl_last_error(const mgl_ctx* ctx);
+
+/* Full multi-level Louvain on one graph. Replaces louvain::run /
+ * louvain_gl::run / louvain_baseline::run (src/louvain/louvain.cu:1401-1715,
+ * louvain_gl.cu:1387-1705, louvain_baseline.cu:1374-1682), which mutate the
+ * caller's graph and print only; here the input is borrowed and the levels,
+ * their modularity, assignments and coarse graphs are returned. */
+mgl_status mgl_louvain(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* o, mgl_result** out);
+
+/* Many independent graphs in one launch sequence (no reference counterpart:
+ * the reference runs one graph per process). results[i] belongs to graphs[i];
+ * each equals what mgl_louvain returns for that graph alone. */
+mgl_status mgl_louvain_batched(mgl_ctx* ctx, const mgl_csr* graphs, int32_t n_graphs,
+                               const mgl_opts* o, mgl_result** results);
+
+/* Result accessors. Pointers stay valid until mgl_result_free. */
+int mgl_result_levels(const mgl_result* r);
+int64_t mgl_result_level_n(const mgl_result* r, int level);
+double mgl_result_level_q(const mgl_result* r, int level);
+int mgl_result_level_iters(const mgl_result* r, int level);
+const double* mgl_result_level_qtraj(const mgl_result* r, int level); /* [iters] */
+const int32_t* mgl_result_level_assign(const mgl_result* r, int level); /* NULL unless keep_levels */
+/* Coarse graph that is the INPUT of `level` (level >= 1); weights as doubles in
+ * the graph's units (exact mode: integers or dyadic values). NULL unless keep_graphs. */
+mgl_status mgl_result_level_graph(const mgl_result* r, int level, int64_t* n, int64_t* nnz,
+                                  const int64_t** offsets, const int32_t** cols,
+                                  const double** w);
+const int32_t* mgl_result_final_assign(const mgl_result* r); /* dense labels of level-0 vertices */
+double mgl_result_final_q(const mgl_result* r);
+int mgl_result_precision(const mgl_result* r); /* precision actually used */
+double mgl_result_seconds(const mgl_result* r); /* device wall time of the run */
+void mgl_result_free(mgl_result* r);
+
+/* ---- Device-resident session: one graph kept in HBM, driven pass by pass.
+ * Used to time the hot path with inputs already resident, and to check each
+ * stage on its own. */
+typedef struct {
+  int64_t n, nnz;
+  double move_ms;      /* last pass: move kernels */
+  double update_ms;    /* last pass: community-weight update */
+  double modularity_ms;/* last pass: sum_in / sum a^2 reduction (+ E-pass when not fused) */
+  double pass_ms;      /* last pass: whole inner-loop body incl. commit */
+  double dominant_ms;  /* last pass: duration of the longest move-kernel launch */
+  int dominant_bin;    /* bin id of that launch */
+  int64_t dominant_vertices, dominant_entries; /* vertices / CSR entries that launch processed */
+  int64_t moved;       /* vertices whose proposal differs from their community */
+  int64_t distinct;    /* (vertex, neighbour community) pairs aggregated in the move */
+  double q;            /* committed modularity */
+} mgl_pass_stats;
+
+mgl_status mgl_session_create(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* o,
+                              mgl_session** s);
+void mgl_session_destroy(mgl_session* s);
+/* Singleton state of level 0: c = identity, a = k, Q computed
+ * (louvain.cu:1492-1541). */
+mgl_status mgl_session_reset(mgl_session* s, double* q0);
+/* One inner-loop iteration of louvain.cu:1548-1660: move, weight update,
+ * modularity, commit-or-reject. *accepted = 1 when committed. */
+mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted);
+mgl_status mgl_session_stats(const mgl_session* s, mgl_pass_stats* st);
+/* Stage entry points on the session's level-0 graph. */
+/* Replaces BIN::bin_create (src/bin/BIN.cu:468-499): returns the bin sizes. */
+mgl_status mgl_session_bins(mgl_session* s, int64_t* bin_sizes, int* n_bins);
+/* Sets the committed assignment c (host array [n]) and recomputes a. */
+mgl_status mgl_session_set_assign(mgl_session* s, const int32_t* c);
+/* Replaces calculate_eicj_and_move_vertex_bin (louvain.cu:1032-1296):
+ * proposal from the committed state into c_new[n]. */
+mgl_status mgl_session_move(mgl_session* s, int up, int32_t* c_new);
+/* Replaces compute_modularity (louvain.cu:105-238) for the committed c. */
+mgl_status mgl_session_modularity(mgl_session* s, double* q);
+/* Community totals a[n] (as doubles, in graph units) of the committed state. */
+mgl_status mgl_session_get_weights(mgl_session* s, double* a, double* k);
+/* Replaces compute_community_weight(_local_atomic) (louvain.cu:1299-1396,
+ * louvain_baseline.cu:1319-1372): applies the moves c -> c_new (host [n]) to a,
+ * then makes c_new the committed assignment. */
+mgl_status mgl_session_apply(mgl_session* s, const int32_t* c_new);
+/* Replaces coarsen_graph_mg::coarsen_graph (coarsen_graph_mg.cu:2054-2240):
+ * coarsens by the committed c. Output buffers sized n+1 / nnz by the caller;
+ * rank[n] receives the dense rank of each community id (-1 if unused). */
+mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64_t* off2,
+                               int32_t* col2, double* w2, int32_t* rank);
+
+/* ---- Synthetic graphs (host, multithreaded). Output CSR is symmetric, without
+ * self-loops or duplicates, rows sorted; free with mgl_csr_free. */
+typedef struct {
+  int64_t n, nnz;
+  int64_t* offsets;
+  int32_t* cols;
+  float* w; /* NULL for unit weights */
+} mgl_host_csr;
+/* weights: 0 unit, 1 uniform (0.5,1.5] (1.5 - U[0,1) in float), 2 lognormal(0,1) as float */
+mgl_status mgl_gen_rmat(int scale, double edge_factor, double a, double b, double c,
+                        uint64_t seed, int weights, int64_t n_limit, mgl_host_csr* out);
+/* Planted partition: n vertices in n_comm communities whose sizes follow a
+ * power law (exponent, min size), mean intra / inter degree. */
+mgl_status mgl_gen_sbm(int64_t n, int64_t n_comm, double exponent, int64_t min_size,
+                       double deg_in, double deg_out, uint64_t seed, int weights,
+                       mgl_host_csr* out);
+/* Reads a Matrix Market file with the reference loader's semantics
+ * (src/mmio/mmio_wrapper.cu:76-295): "symmetric" banner = unit weights, every
+ * off-diagonal entry mirrored, diagonal kept once; "general" banner = only
+ * entries with col < row kept, weights parsed (1.0 if absent), mirrored.
+ * Rows are sorted by column (the reference's order within a row is not fixed). */
+mgl_status mgl_read_mtx(const char* path, mgl_host_csr* out);
+/* Writes assignment / CSR files (text). */
+mgl_status mgl_write_assign(const char* path, const int32_t* assign, int64_t n);
+void mgl_csr_free(mgl_host_csr* g);
+
+/* Build-time / tuning helpers. */
+int mgl_version_major(void);
+/* Number of move bins and their upper degree bounds (inclusive). */
+int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins);
+/* Times the move pass of g under candidate bin layouts and writes the fastest
+ * to path (JSON lines). */
+mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGL_H */

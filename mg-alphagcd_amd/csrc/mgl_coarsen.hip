@@ -1,0 +1,381 @@
+// Graph coarsening by sort + segmented reduce (gfx950).
+//
+// Reference behaviour (src/coarsen_graph/coarsen_graph_mg.cu):
+//   renumbering : flag every used community id, inclusive scan, id <- scan-1,
+//                 i.e. the dense rank in ascending old id (:211-226, :267-286,
+//                 :1330-1363)
+//   aggregation : coarse entry (rank c[u], rank c[v]) carries the sum of the
+//                 weights of all fine entries between the two communities; the
+//                 intra-community entries become the self-loop (:816-917,
+//                 :919-1020, :1022-1125); install :2224-2228
+// The reference builds each coarse row with a per-community hash and emits it
+// in slot order; here every fine entry becomes a 64-bit key
+// (row << b | col), the keys are radix-sorted (own LSD sort, 8-bit digits,
+// stable) and equal keys are reduced, which yields rows sorted by column.
+#include "mgl_kernels.h"
+#include <type_traits>
+
+static unsigned gridc(int64_t n, int per, unsigned cap) {
+  int64_t g = (n + per - 1) / per;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (int32/int64 in -> int64 out, n+1 outputs: out[n] = total).
+#define SCAN_T 256
+#define SCAN_I 8
+#define SCAN_TILE (SCAN_T * SCAN_I)
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t x) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int lo = __shfl_up((int)(uint32_t)x, d, 64);
+    int hi = __shfl_up((int)(uint32_t)((uint64_t)x >> 32), d, 64);
+    int64_t y = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    if (l >= d) x += y;
+  }
+  return x;
+}
+
+// block-wide exclusive scan of one value per thread; returns the block total
+__device__ __forceinline__ int64_t block_excl_scan(int64_t x, int64_t& total) {
+  __shared__ int64_t ws[SCAN_T / 64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  int64_t inc = wave_incl_scan(x);
+  if (l == 63) ws[w] = inc;
+  __syncthreads();
+  int64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_T / 64; ++i) {
+    if (i < w) pre += ws[i];
+    tot += ws[i];
+  }
+  __syncthreads();
+  total = tot;
+  return pre + inc - x;
+}
+
+template <class T>
+__global__ __launch_bounds__(SCAN_T) void k_tile_sums(const T* __restrict__ in, int64_t n,
+                                                      int64_t* __restrict__ sums) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_I; ++i) {
+    const int64_t j = base + (int64_t)i * SCAN_T + threadIdx.x;
+    if (j < n) s += (int64_t)in[j];
+  }
+  int64_t tot;
+  (void)block_excl_scan(s, tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+template <class T>
+__global__ __launch_bounds__(SCAN_T) void k_tile_scan(const T* __restrict__ in, int64_t n,
+                                                      const int64_t* __restrict__ pref,
+                                                      int64_t* __restrict__ out) {
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_I;
+  int64_t v[SCAN_I];
+  int64_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_I; ++i) {
+    const int64_t j = base + i;
+    v[i] = j < n ? (int64_t)in[j] : 0;
+    s += v[i];
+  }
+  int64_t tot;
+  int64_t run = block_excl_scan(s, tot) + (pref ? pref[blockIdx.x] : 0);
+#pragma unroll
+  for (int i = 0; i < SCAN_I; ++i) {
+    const int64_t j = base + i;
+    if (j < n) out[j] = run;
+    run += v[i];
+  }
+  // the thread that owns element n-1 writes the total after it
+  if (base <= n - 1 && n - 1 < base + SCAN_I) out[n] = run;
+}
+
+size_t DevScan::temp_bytes(int64_t n) {
+  size_t b = 0;
+  int64_t m = n;
+  while (m > SCAN_TILE) {
+    const int64_t nb = (m + SCAN_TILE - 1) / SCAN_TILE;
+    b += (size_t)(nb + 1) * 8 * 2;
+    m = nb;
+  }
+  return b + 256;
+}
+
+template <class T>
+static void scan_rec(const T* in, int64_t* out, int64_t n, int64_t* temp, hipStream_t st) {
+  if (n <= 0) {
+    launch_fill_bytes0(out, 8, st);
+    return;
+  }
+  const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb == 1) {
+    k_tile_scan<T><<<1, SCAN_T, 0, st>>>(in, n, nullptr, out);
+    return;
+  }
+  int64_t* sums = temp;
+  int64_t* pref = temp + (nb + 1);
+  k_tile_sums<T><<<(unsigned)nb, SCAN_T, 0, st>>>(in, n, sums);
+  scan_rec<int64_t>(sums, pref, nb, temp + 2 * (nb + 1), st);
+  k_tile_scan<T><<<(unsigned)nb, SCAN_T, 0, st>>>(in, n, pref, out);
+}
+
+void scan_exclusive_i32(const int32_t* in, int64_t* out, int64_t n, void* temp, hipStream_t st) {
+  scan_rec<int32_t>(in, out, n, (int64_t*)temp, st);
+}
+void scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t n, void* temp, hipStream_t st) {
+  scan_rec<int64_t>(in, out, n, (int64_t*)temp, st);
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort of (u64 key, value) pairs, 8-bit digits.
+#define RS_T 256
+#define RS_I 16
+#define RS_TILE (RS_T * RS_I)
+
+__global__ __launch_bounds__(RS_T) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n,
+                                                  int shift, int64_t nblk,
+                                                  int64_t* __restrict__ hist) {
+  __shared__ unsigned int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+  for (int i = 0; i < RS_I; ++i) {
+    const int64_t j = base + (int64_t)i * RS_T + threadIdx.x;
+    if (j < n) atomicAdd(&h[(keys[j] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+template <class VT, bool HAS_V>
+__global__ __launch_bounds__(RS_T) void k_rs_scatter(const uint64_t* __restrict__ kin,
+                                                     const VT* __restrict__ vin, int64_t n,
+                                                     int shift, int64_t nblk,
+                                                     const int64_t* __restrict__ offs,
+                                                     uint64_t* __restrict__ kout,
+                                                     VT* __restrict__ vout) {
+  __shared__ int64_t gbase[256];
+  __shared__ unsigned int run[256];
+  __shared__ unsigned int wc[RS_T / 64][256];
+  gbase[threadIdx.x] = offs[(int64_t)threadIdx.x * nblk + blockIdx.x];
+  run[threadIdx.x] = 0;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const uint64_t lt = (l == 0) ? 0ull : (~0ull >> (64 - l));
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_I; ++r) {
+#pragma unroll
+    for (int i = 0; i < RS_T / 64; ++i) wc[i][threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t j = base + (int64_t)r * RS_T + threadIdx.x;
+    const bool ok = j < n;
+    uint64_t key = ok ? kin[j] : 0;
+    VT val{};
+    if (HAS_V && ok) val = vin[j];
+    const unsigned d = (unsigned)((key >> shift) & 255u);
+    // lanes of this wave holding the same digit
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const uint64_t bmask = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? bmask : ~bmask;
+    }
+    const unsigned rank_w = (unsigned)__popcll(peers & lt);
+    if (ok && rank_w == 0) wc[w][d] = (unsigned)__popcll(peers);
+    __syncthreads();
+    {
+      // thread t owns digit t: prefix over waves, then advance the running count
+      const unsigned t = threadIdx.x;
+      unsigned acc = run[t];
+#pragma unroll
+      for (int i = 0; i < RS_T / 64; ++i) {
+        const unsigned c = wc[i][t];
+        wc[i][t] = acc;
+        acc += c;
+      }
+      run[t] = acc;
+    }
+    __syncthreads();
+    if (ok) {
+      const int64_t pos = gbase[d] + wc[w][d] + rank_w;
+      kout[pos] = key;
+      if (HAS_V) vout[pos] = val;
+    }
+    __syncthreads();
+  }
+}
+
+template <class VT, bool HAS_V>
+static uint64_t* radix_sort(uint64_t* k0, uint64_t* k1, VT* v0, VT* v1, int64_t n, int bits,
+                            int64_t* hist, void* scan_tmp, VT** vres, hipStream_t st) {
+  uint64_t* ki = k0;
+  uint64_t* ko = k1;
+  VT* vi = v0;
+  VT* vo = v1;
+  const int64_t nblk = (n + RS_TILE - 1) / RS_TILE;
+  for (int shift = 0; shift < bits; shift += 8) {
+    k_rs_hist<<<(unsigned)nblk, RS_T, 0, st>>>(ki, n, shift, nblk, hist);
+    int64_t* offs = hist + 256 * nblk + 1;
+    scan_exclusive_i64(hist, offs, 256 * nblk, scan_tmp, st);
+    k_rs_scatter<VT, HAS_V><<<(unsigned)nblk, RS_T, 0, st>>>(ki, vi, n, shift, nblk, offs, ko, vo);
+    uint64_t* tk = ki; ki = ko; ko = tk;
+    VT* tv = vi; vi = vo; vo = tv;
+  }
+  *vres = vi;
+  return ki;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_flag(int64_t n, const vid* __restrict__ c, const int32_t* __restrict__ gid,
+                       const int32_t* __restrict__ keep, int32_t* __restrict__ flag) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (keep && !keep[gid ? gid[v] : 0]) continue;
+    flag[c[v]] = 1;
+  }
+}
+
+__global__ void k_newc(int64_t n, const vid* __restrict__ c, const int32_t* __restrict__ gid,
+                       const int32_t* __restrict__ keep, const int64_t* __restrict__ rank,
+                       vid* __restrict__ newc) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (keep && !keep[gid ? gid[v] : 0]) { newc[v] = -1; continue; }
+    newc[v] = (vid)rank[c[v]];
+  }
+}
+
+// keys of every entry of a kept row, written at e + delta[g] (compaction of
+// the kept graphs' entry ranges). A wavefront walks 64 rows at a time.
+template <class M, class VT>
+__global__ __launch_bounds__(256) void k_keys(int64_t n, const eid* __restrict__ off,
+                                              const vid* __restrict__ col, const void* w,
+                                              const vid* __restrict__ newc,
+                                              const int32_t* __restrict__ gid,
+                                              const int64_t* __restrict__ delta, int bits,
+                                              uint64_t* __restrict__ keys, VT* __restrict__ vals) {
+  // one wavefront per row for long rows is handled by striding lanes over
+  // entries; rows are visited wave-by-wave
+  const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t nw = ((int64_t)gridDim.x * 256) >> 6;
+  for (int64_t v = wave; v < n; v += nw) {
+    const vid r = newc[v];
+    if (r < 0) continue;
+    const int64_t dl = delta ? delta[gid ? gid[v] : 0] : 0;
+    const eid e0 = off[v], e1 = off[v + 1];
+    for (eid e = e0 + lane; e < e1; e += 64) {
+      const uint64_t cc = (uint64_t)(uint32_t)newc[col[e]];
+      keys[e + dl] = ((uint64_t)(uint32_t)r << bits) | cc;
+      if (M::HAS_W) vals[e + dl] = (VT)((const typename M::W*)w)[e];
+    }
+  }
+}
+
+__global__ void k_heads(const uint64_t* __restrict__ k, int64_t n, int32_t* __restrict__ head) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    head[i] = (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
+}
+
+template <class VT, class OT, bool HAS_V>
+__global__ void k_emit(const uint64_t* __restrict__ k, const VT* __restrict__ v, int64_t n,
+                       const int32_t* __restrict__ head, const int64_t* __restrict__ uidx,
+                       int bits, vid* __restrict__ col2, OT* __restrict__ w2,
+                       int64_t* __restrict__ rowcnt) {
+  const uint64_t mask = (bits >= 64) ? ~0ull : ((1ull << bits) - 1ull);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t u = uidx[i + 1] - 1;
+    if (head[i]) {
+      col2[u] = (vid)(k[i] & mask);
+      atomicAdd((unsigned long long*)&rowcnt[k[i] >> bits], 1ull);
+    }
+    const OT x = HAS_V ? (OT)v[i] : (OT)1;
+    if constexpr (std::is_same_v<OT, int64_t>)
+      atomicAdd((unsigned long long*)&w2[u], (unsigned long long)x);
+    else
+      atomicAdd(&w2[u], x);
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <class M, class VT, bool HAS_V, class OT>
+static int coarsen_t(int64_t n, const eid* off, const vid* col, const void* w, const vid* c,
+                     const int32_t* gid, const int32_t* keep, const int64_t* delta,
+                     int64_t e_keep, CoarsenBufs& B, eid* off2, vid* col2, void* w2,
+                     int64_t* C_out, int64_t* E2_out, hipStream_t st) {
+  launch_fill_bytes0(B.flag, sizeof(int32_t) * (size_t)n, st);
+  k_flag<<<gridc(n, 256, 8192), 256, 0, st>>>(n, c, gid, keep, B.flag);
+  scan_exclusive_i32(B.flag, B.rank, n, B.scan_tmp, st);
+  int64_t C = 0;
+  if (hipMemcpyAsync(&C, B.rank + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+  if (hipStreamSynchronize(st) != hipSuccess) return -1;
+  k_newc<<<gridc(n, 256, 8192), 256, 0, st>>>(n, c, gid, keep, B.rank, B.newc);
+  int bits = 1;
+  while ((1ll << bits) < C) ++bits;
+  const int kbits = 2 * bits;
+  k_keys<M, VT><<<gridc(n * 64, 256, 1u << 20), 256, 0, st>>>(n, off, col, w, B.newc, gid, delta,
+                                                             bits, B.k0, (VT*)B.v0);
+  VT* vs = nullptr;
+  uint64_t* ks = B.k0;
+  if (e_keep > 0) {
+    const int64_t need = 256 * ((e_keep + RS_TILE - 1) / RS_TILE) * 2 + 2;
+    if (need > B.hist_cap) return -2;
+    ks = radix_sort<VT, HAS_V>(B.k0, B.k1, (VT*)B.v0, (VT*)B.v1, e_keep, kbits, B.hist,
+                               B.scan_tmp, &vs, st);
+  }
+  int64_t U = 0;
+  if (e_keep > 0) {
+    k_heads<<<gridc(e_keep, 256, 8192), 256, 0, st>>>(ks, e_keep, B.head);
+    // uidx[0] = 0, uidx[i+1] = inclusive count
+    scan_exclusive_i32(B.head, B.uidx, e_keep, B.scan_tmp, st);
+    if (hipMemcpyAsync(&U, B.uidx + e_keep, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return -1;
+  // exclusive scan gives uidx[i] = count before i; emit needs the inclusive
+  // count at i, i.e. uidx[i+1]; out[e_keep] = total makes that valid for all i
+  launch_fill_bytes0(B.rowcnt, sizeof(int64_t) * (size_t)(C + 1), st);
+  launch_fill_bytes0(w2, sizeof(OT) * (size_t)(U > 0 ? U : 1), st);
+  if (e_keep > 0)
+    k_emit<VT, OT, HAS_V><<<gridc(e_keep, 256, 8192), 256, 0, st>>>(ks, vs, e_keep, B.head,
+                                                                   B.uidx, bits, col2, (OT*)w2,
+                                                                   B.rowcnt);
+  scan_exclusive_i64(B.rowcnt, off2, C, B.scan_tmp, st);
+  *C_out = C;
+  *E2_out = U;
+  return 0;
+}
+
+int coarsen_level(int mode_in, int mode_out, int64_t n, const eid* off, const vid* col,
+                  const void* w, const vid* c, const int32_t* gid, const int32_t* keep,
+                  const int64_t* delta, int64_t e_keep, CoarsenBufs& B, eid* off2, vid* col2,
+                  void* w2, int64_t* C_out, int64_t* E2_out, hipStream_t st) {
+  (void)mode_out;
+  switch (mode_in) {
+    case MODE_UNIT:
+      return coarsen_t<ModeUnit, int64_t, false, int64_t>(n, off, col, w, c, gid, keep, delta,
+                                                          e_keep, B, off2, col2, w2, C_out,
+                                                          E2_out, st);
+    case MODE_INT:
+      return coarsen_t<ModeInt, int64_t, true, int64_t>(n, off, col, w, c, gid, keep, delta,
+                                                        e_keep, B, off2, col2, w2, C_out, E2_out,
+                                                        st);
+    case MODE_F64:
+      return coarsen_t<ModeF64, double, true, double>(n, off, col, w, c, gid, keep, delta,
+                                                      e_keep, B, off2, col2, w2, C_out, E2_out,
+                                                      st);
+    case MODE_F32:
+      return coarsen_t<ModeF32, float, true, float>(n, off, col, w, c, gid, keep, delta, e_keep,
+                                                    B, off2, col2, w2, C_out, E2_out, st);
+  }
+  return -1;
+}

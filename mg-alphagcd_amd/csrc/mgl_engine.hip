@@ -1,0 +1,1652 @@
+// Host engine of the MI355X Louvain path: level loop, device-side inner-loop
+// control, coarsening between levels, batching, sessions and the C ABI.
+//
+// Reference driver: src/louvain/louvain.cu:1401-1715 (louvain::run), the same
+// loop in louvain_gl.cu:1387-1705 and louvain_baseline.cu:1374-1682.
+//   phase loop   : while phase < max_phases && Q - Q_old > thr   (:1473)
+//   level start  : c = id, k = row sums, a = k, Q(singletons)    (:1492-1533)
+//   inner loop   : while new_Q - cur_Q > thr: move, flip up/down, update a,
+//                  Q(c_new), commit if the gain exceeds thr, else reject (:1548-1660)
+//   level end    : stop if the level gained <= thr, else coarsen (:1674-1679)
+//
+// Differences of form (results are the same):
+//   * Every decision of the inner loop is taken on the device (one thread per
+//     graph), so a batch of graphs runs the same launch sequence and the host
+//     only polls an "any graph active" word.
+//   * Fused modularity (default): sum_in of proposal p_t is produced by the
+//     move pass that computes p_{t+1}, which reads the same c[u] gathers. One
+//     CSR pass per iteration instead of two; the extra move of the final,
+//     rejected iteration replaces the reference's modularity pass.
+#include "mgl_kernels.h"
+#include "../../include/mgl.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+// ---------------------------------------------------------------------------
+struct mgl_ctx {
+  int device = 0;
+  hipStream_t st = nullptr;
+  std::string err;
+  int lds_max = 65536;
+  BinLayout user_bins;
+  bool have_user_bins = false;
+};
+
+#define FAIL(ctx, code, ...)                                    \
+  do {                                                          \
+    char _b[512];                                               \
+    snprintf(_b, sizeof(_b), __VA_ARGS__);                      \
+    (ctx)->err = _b;                                            \
+    return (code);                                              \
+  } while (0)
+
+#define HIPC(ctx, call)                                                                 \
+  do {                                                                                  \
+    hipError_t _e = (call);                                                             \
+    if (_e != hipSuccess) {                                                             \
+      (void)hipGetLastError();                                                          \
+      FAIL(ctx, _e == hipErrorOutOfMemory ? MGL_OOM : MGL_DEVICE_ERROR, "%s: %s (%s:%d)", \
+           #call, hipGetErrorString(_e), __FILE__, __LINE__);                           \
+    }                                                                                   \
+  } while (0)
+
+// Device buffer owned by the engine.
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    release();
+    if (b == 0) b = 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    else p = nullptr;
+    return e;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+static size_t wsize(int mode) {  // bytes of one stored edge weight
+  switch (mode) {
+    case MODE_UNIT: return 0;
+    case MODE_INT: return 8;
+    case MODE_F64: return 8;
+    case MODE_F32: return 4;
+  }
+  return 8;
+}
+static bool mode_exact(int mode) { return mode == MODE_UNIT || mode == MODE_INT; }
+static int coarse_mode(int mode) { return mode == MODE_UNIT ? MODE_INT : mode; }
+static int slot_bytes(int mode) {
+  switch (mode) {
+    case MODE_UNIT: return 4;
+    case MODE_INT: return 8;
+    case MODE_F64: return 8;
+    case MODE_F32: return 4;
+  }
+  return 8;
+}
+
+// Bin layout for a mode, with LDS tables shrunk to what the device allows.
+static BinLayout layout_for(const mgl_ctx* ctx, int mode) {
+  BinLayout L = ctx->have_user_bins ? ctx->user_bins : default_bins();
+  for (int i = 0; i < L.nb; ++i) {
+    BinSpec& b = L.b[i];
+    if (b.shape == SHAPE_BLOCK) {
+      while (b.lg > 4 &&
+             (size_t)(4 + slot_bytes(mode)) * ((size_t)1 << b.lg) + 2048 > (size_t)ctx->lds_max)
+        --b.lg;
+      if (b.upper > ((int64_t)1 << b.lg)) b.upper = (int64_t)1 << b.lg;
+    }
+  }
+  // keep bounds increasing after any clamp and end with an unbounded bin
+  for (int i = 1; i < L.nb; ++i)
+    if (L.b[i].upper < L.b[i - 1].upper) L.b[i].upper = L.b[i - 1].upper;
+  L.b[L.nb - 1].upper = INT64_MAX;
+  return L;
+}
+
+// ---------------------------------------------------------------------------
+// One level's graph (a union of the batch's graphs still running).
+struct LevelGraph {
+  int64_t n = 0, nnz = 0;
+  int mode = MODE_UNIT;
+  DBuf off, col, w, gid;
+  std::vector<int64_t> voff;  // host, per-graph vertex ranges [G+1]
+  std::vector<int64_t> eoff;  // host, per-graph entry ranges [G+1]
+};
+
+struct LevelOut {
+  int64_t n = 0;
+  double q = 0;
+  int iters = 0;
+  std::vector<double> qtraj;
+  std::vector<int32_t> assign;
+  int64_t gn = 0, gnnz = 0;
+  std::vector<int64_t> goff;
+  std::vector<int32_t> gcol;
+  std::vector<double> gw;
+  bool has_graph = false;
+};
+
+struct mgl_result {
+  int precision = 0;
+  double seconds = 0;
+  std::vector<LevelOut> levels;
+  std::vector<int32_t> final_assign;
+  double final_q = 0;
+};
+
+static const int TRAJ_MAX = 4096;
+
+// ---------------------------------------------------------------------------
+// Engine: runs the multi-level loop over a union of G graphs.
+struct Engine {
+  mgl_ctx* ctx;
+  mgl_opts o;
+  int G = 1;
+  int prec = MGL_PREC_EXACT;
+  int scale = 0;          // dyadic scale of level-0 weights (MODE_INT)
+  BinLayout L;            // for the current level's mode
+  // level-0 sizes (capacity of all per-vertex / per-entry buffers)
+  int64_t n0 = 0, nnz0 = 0;
+  std::vector<int64_t> vbase0;  // per-graph level-0 vertex ranges [G+1]
+  // per-vertex state
+  DBuf bc, bp, bn, bk, ba, bvin, bperm, blab, bfin, bgid0;
+  vid* c = nullptr; vid* p = nullptr; vid* nx = nullptr;
+  // per-graph
+  DBuf bgs, brun, btraj, bkeep, bdelta, bgather, bcounts;
+  std::vector<GState> hgs;
+  // bins of the current level
+  std::vector<int64_t> bin_n, bin_off;
+  DBuf btoff, bgkeys, bgvals;
+  // coarsening buffers
+  DBuf cflag, crank, cnewc, ck0, ck1, cv0, cv1, chist, chead, cuidx, crowcnt, cscan;
+  CoarsenBufs CB{};
+  // level graphs (ping-pong)
+  LevelGraph lg[2];
+  int cur = 0;
+  // host-side phase state per graph
+  std::vector<double> Q_host, Q_old;
+  std::vector<int> phase, running;
+  std::vector<mgl_result*> res;
+  // timing
+  hipEvent_t ev[2 * MGL_MAX_BINS + 8];
+  bool ev_ok = false;
+  mgl_pass_stats stats{};
+  bool timing = false;
+
+  explicit Engine(mgl_ctx* c_) : ctx(c_) {
+    for (auto& e : ev) e = nullptr;
+  }
+  ~Engine() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  hipStream_t st() const { return ctx->st; }
+  LevelGraph& LG() { return lg[cur]; }
+
+  mgl_status alloc_state() {
+    const int64_t n = n0 > 0 ? n0 : 1;
+    const int64_t e = nnz0 > 0 ? nnz0 : 1;
+    HIPC(ctx, bc.ensure(4 * n));
+    HIPC(ctx, bp.ensure(4 * n));
+    HIPC(ctx, bn.ensure(4 * n));
+    HIPC(ctx, bk.ensure(8 * n));
+    HIPC(ctx, ba.ensure(8 * n));
+    HIPC(ctx, bvin.ensure(8 * n));
+    HIPC(ctx, bperm.ensure(4 * n));
+    HIPC(ctx, blab.ensure(4 * n));
+    HIPC(ctx, bfin.ensure(4 * n));
+    HIPC(ctx, bgs.ensure(sizeof(GState) * G));
+    HIPC(ctx, brun.ensure(4 * G));
+    HIPC(ctx, bkeep.ensure(4 * G));
+    HIPC(ctx, bdelta.ensure(8 * G));
+    HIPC(ctx, bgather.ensure(8 * (G + 1) * 2));
+    HIPC(ctx, btraj.ensure(sizeof(double) * (size_t)G * TRAJ_MAX));
+    HIPC(ctx, bcounts.ensure(8 * MGL_MAX_BINS * 3));
+    c = bc.as<vid>(); p = bp.as<vid>(); nx = bn.as<vid>();
+    if (!ev_ok) {
+      for (auto& x : ev) HIPC(ctx, hipEventCreate(&x));
+      ev_ok = true;
+    }
+    return MGL_OK;
+  }
+
+  mgl_status alloc_coarsen() {
+    const int64_t n = n0 > 0 ? n0 : 1;
+    const int64_t e = nnz0 > 0 ? nnz0 : 1;
+    HIPC(ctx, cflag.ensure(4 * n));
+    HIPC(ctx, crank.ensure(8 * (n + 1)));
+    HIPC(ctx, cnewc.ensure(4 * n));
+    HIPC(ctx, ck0.ensure(8 * e));
+    HIPC(ctx, ck1.ensure(8 * e));
+    const int64_t nblk = (e + 4095) / 4096;
+    const int64_t hist_cap = 256 * nblk * 2 + 2;
+    HIPC(ctx, chist.ensure(8 * hist_cap));
+    HIPC(ctx, chead.ensure(4 * e));
+    HIPC(ctx, cuidx.ensure(8 * (e + 1)));
+    HIPC(ctx, crowcnt.ensure(8 * (n + 1)));
+    const int64_t smax = std::max<int64_t>(std::max<int64_t>(n + 1, e + 1), 256 * nblk + 1);
+    HIPC(ctx, cscan.ensure(DevScan::temp_bytes(smax)));
+    CB.flag = cflag.as<int32_t>();
+    CB.rank = crank.as<int64_t>();
+    CB.newc = cnewc.as<vid>();
+    CB.k0 = ck0.as<uint64_t>();
+    CB.k1 = ck1.as<uint64_t>();
+    CB.v0 = cv0.p;
+    CB.v1 = cv1.p;
+    CB.hist = chist.as<int64_t>();
+    CB.head = chead.as<int32_t>();
+    CB.uidx = cuidx.as<int64_t>();
+    CB.rowcnt = crowcnt.as<int64_t>();
+    CB.scan_tmp = cscan.p;
+    CB.hist_cap = hist_cap;
+    return MGL_OK;
+  }
+
+  // -------------------------------------------------------------------------
+  // Binning of the current level (BIN::bin_create, BIN.cu:468-499).
+  mgl_status make_bins() {
+    LevelGraph& g = LG();
+    L = layout_for(ctx, g.mode);
+    const int nb = L.nb;
+    bin_n.assign(nb, 0);
+    bin_off.assign(nb + 1, 0);
+    if (g.n == 0) return MGL_OK;
+    unsigned long long* cnt = bcounts.as<unsigned long long>();
+    launch_fill_bytes0(cnt, 8 * MGL_MAX_BINS * 2, st());
+    launch_bin_count(g.off.as<eid>(), g.n, L, cnt, st());
+    std::vector<unsigned long long> h(MGL_MAX_BINS);
+    HIPC(ctx, hipMemcpyAsync(h.data(), cnt, 8 * MGL_MAX_BINS, hipMemcpyDeviceToHost, st()));
+    HIPC(ctx, hipStreamSynchronize(st()));
+    for (int i = 0; i < nb; ++i) {
+      bin_n[i] = (int64_t)h[i];
+      bin_off[i + 1] = bin_off[i] + bin_n[i];
+    }
+    std::vector<unsigned long long> cur0(MGL_MAX_BINS, 0);
+    for (int i = 0; i < nb; ++i) cur0[i] = (unsigned long long)bin_off[i];
+    unsigned long long* cursor = cnt + MGL_MAX_BINS;
+    HIPC(ctx, hipMemcpyAsync(cursor, cur0.data(), 8 * MGL_MAX_BINS, hipMemcpyHostToDevice, st()));
+    HIPC(ctx, hipStreamSynchronize(st()));
+    launch_bin_scatter(g.off.as<eid>(), g.n, L, cursor, bperm.as<vid>(), st());
+    // global-hash tables of the last bin
+    const int last = nb - 1;
+    if (L.b[last].shape == SHAPE_GLOBAL && bin_n[last] > 0) {
+      const int64_t m = bin_n[last];
+      HIPC(ctx, btoff.ensure(8 * (m + 1)));
+      DBuf sizes;
+      HIPC(ctx, sizes.ensure(8 * m));
+      launch_table_sizes(g.off.as<eid>(), bperm.as<vid>() + bin_off[last], m, sizes.as<int64_t>(),
+                         st());
+      DBuf tmp;
+      HIPC(ctx, tmp.ensure(DevScan::temp_bytes(m + 1)));
+      scan_exclusive_i64(sizes.as<int64_t>(), btoff.as<int64_t>(), m, tmp.p, st());
+      int64_t total = 0;
+      HIPC(ctx, hipMemcpyAsync(&total, btoff.as<int64_t>() + m, 8, hipMemcpyDeviceToHost, st()));
+      HIPC(ctx, hipStreamSynchronize(st()));
+      HIPC(ctx, bgkeys.ensure(4 * total));
+      HIPC(ctx, bgvals.ensure((size_t)slot_bytes(g.mode) * total));
+    }
+    return MGL_OK;
+  }
+
+  MoveArgs move_args(const vid* from, vid* to, int up, void* vin) {
+    LevelGraph& g = LG();
+    MoveArgs A;
+    A.off = g.off.as<eid>();
+    A.col = g.col.as<vid>();
+    A.w = g.w.p;
+    A.k = bk.p;
+    A.c = from;
+    A.a = ba.p;
+    A.c_new = to;
+    A.vin = vin;
+    A.gid = g.gid.as<int32_t>();
+    A.gs = bgs.as<GState>();
+    A.up = up;
+    A.version = o.version;
+    A.tie_small = o.tie_rule == MGL_TIE_SMALLEST;
+    A.stats = nullptr;
+    return A;
+  }
+
+  // All move kernels of the level (calculate_eicj_and_move_vertex_bin,
+  // louvain.cu:1032-1296), one stream, largest bins first.
+  mgl_status move(const vid* from, vid* to, int up, void* vin, bool time_bins) {
+    LevelGraph& g = LG();
+    MoveArgs A = move_args(from, to, up, vin);
+    if (timing) A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS;
+    int ne = 0;
+    for (int b = L.nb - 1; b >= 1; --b) {
+      if (bin_n[b] == 0) continue;
+      if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b], st()));
+      const int rc = launch_move_bin(g.mode, L.b[b], A, bperm.as<vid>() + bin_off[b], bin_n[b],
+                                     btoff.as<int64_t>(), bgkeys.as<vid>(), bgvals.p, st());
+      if (rc != 0) FAIL(ctx, MGL_UNSUPPORTED, "no move kernel for bin %d (s=%d lg=%d)", b, L.b[b].s, L.b[b].lg);
+      if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b + 1], st()));
+      ++ne;
+    }
+    HIPC(ctx, hipGetLastError());
+    return MGL_OK;
+  }
+
+  mgl_status rows(int op, const vid* cc, void* kout, void* vin, bool skip_inactive) {
+    LevelGraph& g = LG();
+    RowArgs R;
+    R.off = g.off.as<eid>();
+    R.col = g.col.as<vid>();
+    R.w = g.w.p;
+    R.c = cc;
+    R.k = kout;
+    R.vin = vin;
+    R.gid = g.gid.as<int32_t>();
+    R.gs = skip_inactive ? bgs.as<GState>() : nullptr;
+    for (int b = L.nb - 1; b >= 1; --b)
+      launch_rows_bin(g.mode, op, L.b[b], R, bperm.as<vid>() + bin_off[b], bin_n[b], st());
+    HIPC(ctx, hipGetLastError());
+    return MGL_OK;
+  }
+
+  // a[x] = sum of k over members of x (used after an externally set c).
+  mgl_status recompute_a(const vid* cc);
+
+  // Level start (louvain.cu:1492-1533) for the graphs with running[g] = 1.
+  mgl_status level_start() {
+    LevelGraph& g = LG();
+    std::vector<int32_t> run(G);
+    for (int i = 0; i < G; ++i) run[i] = running[i];
+    HIPC(ctx, hipMemcpyAsync(brun.p, run.data(), 4 * G, hipMemcpyHostToDevice, st()));
+    HIPC(ctx, hipStreamSynchronize(st()));
+    launch_level_begin(G, bgs.as<GState>(), brun.as<int32_t>(), st());
+    mgl_status s = make_bins();
+    if (s) return s;
+    launch_fill_bytes0(bk.p, 8 * (size_t)std::max<int64_t>(g.n, 1), st());
+    launch_fill_bytes0(bvin.p, 8 * (size_t)std::max<int64_t>(g.n, 1), st());
+    s = rows(OP_WEIGHT, nullptr, bk.p, bvin.p, false);
+    if (s) return s;
+    launch_copy_k_to_a(g.mode, g.n, bk.p, ba.p, st());
+    launch_iota(c, g.n, st());
+    launch_iota(p, g.n, st());
+    launch_iota(nx, g.n, st());
+    launch_reduce(g.mode, g.n, bvin.p, ba.p, g.gid.as<int32_t>(), bgs.as<GState>(), st());
+    launch_level_init(G, bgs.as<GState>(), mode_exact(g.mode) ? 1 : 0, st());
+    HIPC(ctx, hipGetLastError());
+    return MGL_OK;
+  }
+
+  // First proposal of a level in fused mode: p = move(c), a = a(p), sum a^2(p).
+  mgl_status pre_move() {
+    LevelGraph& g = LG();
+    mgl_status s = move(c, p, 1, nullptr, false);
+    if (s) return s;
+    launch_update(g.mode, g.n, c, p, bk.p, ba.p, g.gid.as<int32_t>(), bgs.as<GState>(), st());
+    launch_reduce(g.mode, g.n, nullptr, ba.p, g.gid.as<int32_t>(), bgs.as<GState>(), st());
+    launch_decide(G, bgs.as<GState>(), o.threshold, o.max_iter, 1, mode_exact(g.mode), nullptr, 0,
+                  1, st());
+    HIPC(ctx, hipGetLastError());
+    return MGL_OK;
+  }
+
+  // One inner iteration. it = 1-based index of the proposal being decided.
+  mgl_status iteration(int it) {
+    LevelGraph& g = LG();
+    const bool ex = mode_exact(g.mode);
+    const int32_t* gid = g.gid.as<int32_t>();
+    GState* gs = bgs.as<GState>();
+    double* traj = btraj.as<double>();
+    if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 0], st()));
+    if (o.fuse_modularity) {
+      const int up = ((it + 1) & 1) ? 1 : 0;  // direction of proposal it+1
+      mgl_status s = move(p, nx, up, bvin.p, timing);
+      if (s) return s;
+      if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 1], st()));
+      launch_update(g.mode, g.n, p, nx, bk.p, ba.p, gid, gs, st());
+      if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 2], st()));
+      launch_reduce(g.mode, g.n, bvin.p, ba.p, gid, gs, st());
+      launch_decide(G, gs, o.threshold, o.max_iter, 1, ex, traj, TRAJ_MAX, 0, st());
+      if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 3], st()));
+      launch_fixup(g.n, c, p, nx, gid, gs, st());
+      vid* t = c; c = p; p = nx; nx = t;
+    } else {
+      const int up = (it & 1) ? 1 : 0;
+      mgl_status s = move(c, p, up, nullptr, timing);
+      if (s) return s;
+      if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 1], st()));
+      launch_update(g.mode, g.n, c, p, bk.p, ba.p, gid, gs, st());
+      if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 2], st()));
+      s = rows(OP_INTRA, p, nullptr, bvin.p, true);
+      if (s) return s;
+      launch_reduce(g.mode, g.n, bvin.p, ba.p, gid, gs, st());
+      launch_decide(G, gs, o.threshold, o.max_iter, 0, ex, traj, TRAJ_MAX, 0, st());
+      if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 3], st()));
+      launch_fixup(g.n, c, p, nullptr, gid, gs, st());
+    }
+    if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 4], st()));
+    HIPC(ctx, hipGetLastError());
+    return MGL_OK;
+  }
+
+  mgl_status read_gs() {
+    hgs.resize(G);
+    HIPC(ctx, hipMemcpyAsync(hgs.data(), bgs.p, sizeof(GState) * G, hipMemcpyDeviceToHost, st()));
+    HIPC(ctx, hipStreamSynchronize(st()));
+    return MGL_OK;
+  }
+
+  mgl_status any_active(bool& any) {
+    mgl_status s = read_gs();
+    if (s) return s;
+    any = false;
+    for (int i = 0; i < G; ++i) any |= hgs[i].active != 0;
+    return MGL_OK;
+  }
+
+  // Inner loop of a level (louvain.cu:1548-1660).
+  mgl_status run_level() {
+    mgl_status s;
+    if (o.fuse_modularity) {
+      s = pre_move();
+      if (s) return s;
+    }
+    for (int it = 1;; ++it) {
+      if (it > (1 << 22)) FAIL(ctx, MGL_DEVICE_ERROR, "inner loop did not terminate");
+      s = iteration(it);
+      if (s) return s;
+      bool any;
+      s = any_active(any);
+      if (s) return s;
+      if (!any) break;
+    }
+    return MGL_OK;
+  }
+
+  mgl_status level_end(int level);
+  mgl_status run(std::vector<mgl_result*>& out);
+  mgl_status setup_union(const mgl_csr* graphs, int ng);
+};
+
+// ---------------------------------------------------------------------------
+__global__ void k_scatter_a(int64_t n, const vid* c, const int64_t* k, int64_t* a) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd((unsigned long long*)&a[c[v]], (unsigned long long)k[v]);
+}
+__global__ void k_scatter_a_d(int64_t n, const vid* c, const double* k, double* a) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&a[c[v]], k[v]);
+}
+mgl_status Engine::recompute_a(const vid* cc) {
+  LevelGraph& g = LG();
+  launch_fill_bytes0(ba.p, 8 * (size_t)std::max<int64_t>(g.n, 1), st());
+  const unsigned grid = (unsigned)std::min<int64_t>(8192, (g.n + 255) / 256 + 1);
+  if (mode_exact(g.mode))
+    k_scatter_a<<<grid, 256, 0, st()>>>(g.n, cc, bk.as<int64_t>(), ba.as<int64_t>());
+  else
+    k_scatter_a_d<<<grid, 256, 0, st()>>>(g.n, cc, bk.as<double>(), ba.as<double>());
+  HIPC(ctx, hipGetLastError());
+  return MGL_OK;
+}
+
+// label composition kernels
+// fin[v0] = rank_all[c[lab[v0]]] - rank_all[voff[g]]  (dense label inside graph g)
+__global__ void k_label_final(int64_t n0, const vid* lab, const vid* c, const int64_t* rank,
+                              const int32_t* gid0, const int32_t* run, const int64_t* gbase,
+                              vid* fin) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n0;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int g = gid0 ? gid0[v] : 0;
+    if (!run[g]) continue;
+    fin[v] = (vid)(rank[c[lab[v]]] - gbase[g]);
+  }
+}
+// lab[v0] = newc[lab[v0]] for graphs that continue
+__global__ void k_label_next(int64_t n0, vid* lab, const vid* newc, const int32_t* gid0,
+                             const int32_t* keep) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n0;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int g = gid0 ? gid0[v] : 0;
+    if (!keep[g]) continue;
+    lab[v] = newc[lab[v]];
+  }
+}
+__global__ void k_gather_i64(const int64_t* src, const int64_t* idx, int m, int64_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = src[idx[i]];
+}
+__global__ void k_flag_run(int64_t n, const vid* c, const int32_t* gid, const int32_t* run,
+                           int32_t* flag) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    if (!run[gid ? gid[v] : 0]) continue;
+    flag[c[v]] = 1;
+  }
+}
+
+static mgl_status copy_level_graph_to_host(mgl_ctx* ctx, LevelGraph& g, int64_t v0, int64_t v1,
+                                           int scale, LevelOut& lo) {
+  const int64_t e0 = g.eoff.empty() ? 0 : 0;
+  (void)e0;
+  std::vector<int64_t> off(v1 - v0 + 1);
+  HIPC(ctx, hipMemcpyAsync(off.data(), g.off.as<eid>() + v0, 8 * (v1 - v0 + 1),
+                           hipMemcpyDeviceToHost, ctx->st));
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  const int64_t eb = off[0], ee = off[v1 - v0];
+  lo.gn = v1 - v0;
+  lo.gnnz = ee - eb;
+  lo.goff.resize(off.size());
+  for (size_t i = 0; i < off.size(); ++i) lo.goff[i] = off[i] - eb;
+  lo.gcol.resize(lo.gnnz);
+  lo.gw.resize(lo.gnnz);
+  if (lo.gnnz) {
+    HIPC(ctx, hipMemcpyAsync(lo.gcol.data(), g.col.as<vid>() + eb, 4 * lo.gnnz,
+                             hipMemcpyDeviceToHost, ctx->st));
+    if (g.mode == MODE_UNIT) {
+      std::fill(lo.gw.begin(), lo.gw.end(), 1.0);
+    } else if (g.mode == MODE_INT) {
+      std::vector<int64_t> w(lo.gnnz);
+      HIPC(ctx, hipMemcpyAsync(w.data(), g.w.as<int64_t>() + eb, 8 * lo.gnnz,
+                               hipMemcpyDeviceToHost, ctx->st));
+      HIPC(ctx, hipStreamSynchronize(ctx->st));
+      for (int64_t i = 0; i < lo.gnnz; ++i) lo.gw[i] = std::ldexp((double)w[i], -scale);
+    } else if (g.mode == MODE_F64) {
+      HIPC(ctx, hipMemcpyAsync(lo.gw.data(), g.w.as<double>() + eb, 8 * lo.gnnz,
+                               hipMemcpyDeviceToHost, ctx->st));
+    } else {
+      std::vector<float> w(lo.gnnz);
+      HIPC(ctx, hipMemcpyAsync(w.data(), g.w.as<float>() + eb, 4 * lo.gnnz,
+                               hipMemcpyDeviceToHost, ctx->st));
+      HIPC(ctx, hipStreamSynchronize(ctx->st));
+      for (int64_t i = 0; i < lo.gnnz; ++i) lo.gw[i] = w[i];
+    }
+    HIPC(ctx, hipStreamSynchronize(ctx->st));
+    for (int64_t i = 0; i < lo.gnnz; ++i) lo.gcol[i] -= (int32_t)v0;
+  }
+  lo.has_graph = true;
+  return MGL_OK;
+}
+
+// Level end (louvain.cu:1664-1693): record the level, decide per graph whether
+// to coarsen, build the next level's union graph.
+mgl_status Engine::level_end(int level) {
+  LevelGraph& g = LG();
+  mgl_status s = read_gs();
+  if (s) return s;
+  std::vector<double> traj((size_t)G * TRAJ_MAX);
+  HIPC(ctx, hipMemcpyAsync(traj.data(), btraj.p, sizeof(double) * traj.size(),
+                           hipMemcpyDeviceToHost, st()));
+  std::vector<int32_t> hc;
+  if (o.keep_levels && g.n > 0) {
+    hc.resize(g.n);
+    HIPC(ctx, hipMemcpyAsync(hc.data(), c, 4 * g.n, hipMemcpyDeviceToHost, st()));
+  }
+  HIPC(ctx, hipStreamSynchronize(st()));
+  std::vector<int32_t> keep(G, 0);
+  for (int i = 0; i < G; ++i) {
+    if (!running[i]) continue;
+    LevelOut lo;
+    lo.n = g.voff[i + 1] - g.voff[i];
+    lo.q = hgs[i].q_cur;
+    lo.iters = hgs[i].iters;
+    const int nt = std::min(lo.iters, TRAJ_MAX);
+    lo.qtraj.assign(traj.begin() + (size_t)i * TRAJ_MAX, traj.begin() + (size_t)i * TRAJ_MAX + nt);
+    if (o.keep_levels) {
+      lo.assign.resize(lo.n);
+      for (int64_t v = 0; v < lo.n; ++v) lo.assign[v] = hc[g.voff[i] + v] - (int32_t)g.voff[i];
+    }
+    if (o.keep_graphs && level > 0) {
+      s = copy_level_graph_to_host(ctx, g, g.voff[i], g.voff[i + 1], scale, lo);
+      if (s) return s;
+    }
+    res[i]->levels.push_back(std::move(lo));
+    Q_host[i] = hgs[i].q_cur;
+    const bool gained = (Q_host[i] - Q_old[i]) > o.threshold;  // louvain.cu:1674
+    // the reference coarsens, increments the phase and re-tests phase < max_phases;
+    // a coarsening whose level would never run is skipped here
+    keep[i] = (gained && phase[i] + 1 < o.max_phases) ? 1 : 0;
+  }
+  // final labels of every graph that ran this level: dense rank of c within the graph
+  {
+    std::vector<int32_t> run(G);
+    for (int i = 0; i < G; ++i) run[i] = running[i];
+    HIPC(ctx, hipMemcpyAsync(brun.p, run.data(), 4 * G, hipMemcpyHostToDevice, st()));
+    if (g.n > 0) {
+      launch_fill_bytes0(CB.flag, 4 * (size_t)g.n, st());
+      const unsigned gr = (unsigned)std::min<int64_t>(8192, (g.n + 255) / 256 + 1);
+      k_flag_run<<<gr, 256, 0, st()>>>(g.n, c, g.gid.as<int32_t>(), brun.as<int32_t>(), CB.flag);
+      scan_exclusive_i32(CB.flag, CB.rank, g.n, CB.scan_tmp, st());
+      std::vector<int64_t> idx(G);
+      for (int i = 0; i < G; ++i) idx[i] = g.voff[i];
+      int64_t* didx = bgather.as<int64_t>();
+      int64_t* dout = didx + (G + 1);
+      HIPC(ctx, hipMemcpyAsync(didx, idx.data(), 8 * G, hipMemcpyHostToDevice, st()));
+      k_gather_i64<<<(G + 255) / 256, 256, 0, st()>>>(CB.rank, didx, G, dout);
+      const unsigned g0 = (unsigned)std::min<int64_t>(8192, (n0 + 255) / 256 + 1);
+      k_label_final<<<g0, 256, 0, st()>>>(n0, blab.as<vid>(), c, CB.rank, bgid0.as<int32_t>(),
+                                          brun.as<int32_t>(), dout, bfin.as<vid>());
+      HIPC(ctx, hipGetLastError());
+    }
+    HIPC(ctx, hipStreamSynchronize(st()));
+  }
+  bool any_keep = false;
+  for (int i = 0; i < G; ++i) any_keep |= keep[i] != 0;
+  for (int i = 0; i < G; ++i) {
+    if (!running[i]) continue;
+    if (!keep[i]) running[i] = 0;
+    else phase[i]++;
+  }
+  if (!any_keep) return MGL_OK;
+  // ---- coarsen the kept graphs (coarsen_graph_mg::coarsen_graph) ----
+  HIPC(ctx, hipMemcpyAsync(bkeep.p, keep.data(), 4 * G, hipMemcpyHostToDevice, st()));
+  std::vector<int64_t> delta(G, 0);
+  int64_t e_keep = 0;
+  for (int i = 0; i < G; ++i) {
+    if (!keep[i]) continue;
+    delta[i] = e_keep - g.eoff[i];
+    e_keep += g.eoff[i + 1] - g.eoff[i];
+  }
+  HIPC(ctx, hipMemcpyAsync(bdelta.p, delta.data(), 8 * G, hipMemcpyHostToDevice, st()));
+  HIPC(ctx, hipStreamSynchronize(st()));
+  LevelGraph& ng = lg[cur ^ 1];
+  ng.mode = coarse_mode(g.mode);
+  // upper bounds: C <= n, E2 <= e_keep
+  HIPC(ctx, ng.off.ensure(8 * (g.n + 2)));
+  HIPC(ctx, ng.col.ensure(4 * std::max<int64_t>(e_keep, 1)));
+  HIPC(ctx, ng.w.ensure(std::max<size_t>(wsize(ng.mode), 4) * std::max<int64_t>(e_keep, 1)));
+  if (g.mode != MODE_UNIT) {
+    // value ping-pong buffers of the sort, sized by the kept entries
+    HIPC(ctx, cv0.ensure(8 * std::max<int64_t>(e_keep, 1)));
+    HIPC(ctx, cv1.ensure(8 * std::max<int64_t>(e_keep, 1)));
+    CB.v0 = cv0.p;
+    CB.v1 = cv1.p;
+  }
+  int64_t C = 0, E2 = 0;
+  const int rc = coarsen_level(g.mode, ng.mode, g.n, g.off.as<eid>(), g.col.as<vid>(), g.w.p, c,
+                               g.gid.as<int32_t>(), bkeep.as<int32_t>(),
+                               G > 1 ? bdelta.as<int64_t>() : nullptr, e_keep, CB,
+                               ng.off.as<eid>(), ng.col.as<vid>(), ng.w.p, &C, &E2, st());
+  if (rc != 0) FAIL(ctx, MGL_DEVICE_ERROR, "coarsening failed (%d)", rc);
+  HIPC(ctx, hipGetLastError());
+  ng.n = C;
+  ng.nnz = E2;
+  // per-graph ranges of the coarse union: rank at each graph's first fine vertex
+  std::vector<int64_t> idx(G + 1);
+  for (int i = 0; i <= G; ++i) idx[i] = g.voff[i];
+  int64_t* didx = bgather.as<int64_t>();
+  int64_t* dout = didx + (G + 1);
+  HIPC(ctx, hipMemcpyAsync(didx, idx.data(), 8 * (G + 1), hipMemcpyHostToDevice, st()));
+  k_gather_i64<<<(G + 1 + 255) / 256, 256, 0, st()>>>(CB.rank, didx, G + 1, dout);
+  ng.voff.resize(G + 1);
+  HIPC(ctx, hipMemcpyAsync(ng.voff.data(), dout, 8 * (G + 1), hipMemcpyDeviceToHost, st()));
+  HIPC(ctx, hipStreamSynchronize(st()));
+  // entry ranges: coarse offsets at the graph boundaries
+  HIPC(ctx, hipMemcpyAsync(didx, ng.voff.data(), 8 * (G + 1), hipMemcpyHostToDevice, st()));
+  k_gather_i64<<<(G + 1 + 255) / 256, 256, 0, st()>>>(ng.off.as<int64_t>(), didx, G + 1, dout);
+  ng.eoff.resize(G + 1);
+  HIPC(ctx, hipMemcpyAsync(ng.eoff.data(), dout, 8 * (G + 1), hipMemcpyDeviceToHost, st()));
+  HIPC(ctx, hipStreamSynchronize(st()));
+  if (G > 1) {
+    HIPC(ctx, ng.gid.ensure(4 * std::max<int64_t>(C, 1)));
+    HIPC(ctx, hipMemcpyAsync(didx, ng.voff.data(), 8 * (G + 1), hipMemcpyHostToDevice, st()));
+    launch_gid_fill(didx, G, ng.gid.as<int32_t>(), st());
+  }
+  // labels of continuing graphs move to the coarse ids
+  const unsigned g0 = (unsigned)std::min<int64_t>(8192, (n0 + 255) / 256 + 1);
+  k_label_next<<<g0, 256, 0, st()>>>(n0, blab.as<vid>(), CB.newc, bgid0.as<int32_t>(),
+                                     bkeep.as<int32_t>());
+  HIPC(ctx, hipGetLastError());
+  cur ^= 1;
+  return MGL_OK;
+}
+
+mgl_status Engine::run(std::vector<mgl_result*>& out) {
+  auto t0 = std::chrono::steady_clock::now();
+  Q_host.assign(G, 0.0);
+  Q_old.assign(G, -1.0);
+  phase.assign(G, 0);
+  running.assign(G, 1);
+  launch_iota(blab.as<vid>(), n0, st());
+  int level = 0;
+  while (true) {
+    bool any = false;
+    for (int i = 0; i < G; ++i) {
+      // louvain.cu:1473: phase < max_phases && Q - Q_old > thr
+      if (running[i] && !(phase[i] < o.max_phases && (Q_host[i] - Q_old[i]) > o.threshold))
+        running[i] = 0;
+      any |= running[i] != 0;
+    }
+    if (!any) break;
+    for (int i = 0; i < G; ++i)
+      if (running[i]) Q_old[i] = Q_host[i];
+    mgl_status s = level_start();
+    if (s) return s;
+    s = run_level();
+    if (s) return s;
+    s = level_end(level);
+    if (s) return s;
+    ++level;
+  }
+  // final labels
+  std::vector<int32_t> fin(n0);
+  if (n0) {
+    HIPC(ctx, hipMemcpyAsync(fin.data(), bfin.p, 4 * n0, hipMemcpyDeviceToHost, st()));
+  }
+  HIPC(ctx, hipStreamSynchronize(st()));
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const LevelGraph& l0 = lg[0];
+  (void)l0;
+  for (int i = 0; i < G; ++i) {
+    mgl_result* r = out[i];
+    const int64_t b = vbase0[i], e = vbase0[i + 1];
+    r->final_assign.assign(fin.begin() + b, fin.begin() + e);
+    r->final_q = r->levels.empty() ? 0.0 : r->levels.back().q;
+    r->seconds = secs;
+    r->precision = prec;
+  }
+  return MGL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Host helpers
+#include <thread>
+
+template <class F>
+static void parallel_for(int64_t n, F f) {
+  int nt = (int)std::thread::hardware_concurrency();
+  if (nt < 1) nt = 1;
+  if (nt > 16) nt = 16;
+  if (n < 1 << 16) nt = 1;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([=]() {
+      const int64_t b = n * t / nt, e = n * (t + 1) / nt;
+      f(b, e, t);
+    });
+  for (auto& x : th) x.join();
+}
+
+// Smallest S >= 0 such that every weight * 2^S is an integer; -1 if a weight is
+// not finite and positive or the scaled total would exceed 2^62.
+static int dyadic_scale(const float* w, int64_t nnz) {
+  int nt = 16;
+  std::vector<int> sc(nt, 0);
+  std::vector<int> bad(nt, 0);
+  std::vector<long double> tot(nt, 0);
+  parallel_for(nnz, [&](int64_t b, int64_t e, int t) {
+    int S = 0;
+    long double s = 0;
+    for (int64_t i = b; i < e; ++i) {
+      const float x = w[i];
+      if (!(x > 0.0f) || !std::isfinite(x)) { bad[t] = 1; return; }
+      int ex;
+      const double m = std::frexp((double)x, &ex);
+      const uint64_t mi = (uint64_t)std::ldexp(m, 53);
+      const int need = 53 - __builtin_ctzll(mi) - ex;
+      if (need > S) S = need;
+      s += x;
+    }
+    sc[t] = S;
+    tot[t] = s;
+  });
+  int S = 0;
+  long double T = 0;
+  for (int t = 0; t < nt; ++t) {
+    if (bad[t]) return -1;
+    S = std::max(S, sc[t]);
+    T += tot[t];
+  }
+  if (T * std::ldexp(1.0L, S) > std::ldexp(1.0L, 62)) return -1;
+  return S;
+}
+
+__global__ void k_add_i64(int64_t* x, int64_t n, int64_t add) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] += add;
+}
+__global__ void k_add_i32(int32_t* x, int64_t n, int32_t add) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] += add;
+}
+__global__ void k_fill_one(int to_mode, void* out, int64_t nnz) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    if (to_mode == MODE_F64) ((double*)out)[i] = 1.0;
+    else ((float*)out)[i] = 1.0f;
+  }
+}
+static unsigned grid_n(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > 8192) g = 8192;
+  return (unsigned)g;
+}
+
+static mgl_status validate_csr(mgl_ctx* ctx, const mgl_csr* g, int level) {
+  if (g->n < 0 || g->nnz < 0) FAIL(ctx, MGL_INVALID_ARG, "negative size");
+  if (g->n > 0x7fffffffLL) FAIL(ctx, MGL_UNSUPPORTED, "more than 2^31-1 vertices");
+  if (g->n > 0 && (!g->offsets || (g->nnz > 0 && !g->cols)))
+    FAIL(ctx, MGL_INVALID_ARG, "null CSR array");
+  if (g->n == 0) {
+    if (g->nnz != 0) FAIL(ctx, MGL_INVALID_GRAPH, "nnz > 0 with no vertices");
+    return MGL_OK;
+  }
+  if (g->offsets[0] != 0 || g->offsets[g->n] != g->nnz)
+    FAIL(ctx, MGL_INVALID_GRAPH, "offsets[0] must be 0 and offsets[n] must equal nnz");
+  if (level < 1) return MGL_OK;
+  std::vector<int> bad(16, 0);
+  const int64_t n = g->n;
+  parallel_for(n, [&](int64_t b, int64_t e, int t) {
+    for (int64_t v = b; v < e; ++v) {
+      if (g->offsets[v + 1] < g->offsets[v]) { bad[t] = 1; return; }
+    }
+  });
+  for (int x : bad)
+    if (x) FAIL(ctx, MGL_INVALID_GRAPH, "offsets are not non-decreasing");
+  parallel_for(g->nnz, [&](int64_t b, int64_t e, int t) {
+    for (int64_t i = b; i < e; ++i) {
+      const int32_t x = g->cols[i];
+      if (x < 0 || x >= n) { bad[t] = 2; return; }
+      if (g->w && !(g->w[i] >= 0.0f && std::isfinite(g->w[i]))) { bad[t] = 3; return; }
+    }
+  });
+  for (int x : bad) {
+    if (x == 2) FAIL(ctx, MGL_INVALID_GRAPH, "column index out of range");
+    if (x == 3) FAIL(ctx, MGL_INVALID_GRAPH, "edge weight negative or not finite");
+  }
+  if (level >= 2) {
+    // symmetry: every (u,v,w) has a matching (v,u,w)
+    std::vector<int> asym(16, 0);
+    parallel_for(n, [&](int64_t b, int64_t e, int t) {
+      for (int64_t u = b; u < e && !asym[t]; ++u)
+        for (int64_t i = g->offsets[u]; i < g->offsets[u + 1]; ++i) {
+          const int32_t v = g->cols[i];
+          bool found = false;
+          for (int64_t j = g->offsets[v]; j < g->offsets[v + 1]; ++j)
+            if (g->cols[j] == (int32_t)u && (!g->w || g->w[j] == g->w[i])) { found = true; break; }
+          if (!found) { asym[t] = 1; break; }
+        }
+    });
+    for (int x : asym)
+      if (x) FAIL(ctx, MGL_INVALID_GRAPH, "graph is not symmetric");
+  }
+  return MGL_OK;
+}
+
+mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
+  G = ng;
+  if (G < 1) FAIL(ctx, MGL_INVALID_ARG, "no graphs");
+  for (int i = 0; i < G; ++i) {
+    mgl_status s = validate_csr(ctx, &graphs[i], o.validate);
+    if (s) return s;
+  }
+  bool weighted = false;
+  for (int i = 0; i < G; ++i) weighted |= graphs[i].w != nullptr;
+  prec = o.precision;
+  int S = 0;
+  if (weighted && (prec == MGL_PREC_AUTO || prec == MGL_PREC_EXACT)) {
+    for (int i = 0; i < G && S >= 0; ++i) {
+      if (!graphs[i].w) continue;
+      const int si = dyadic_scale(graphs[i].w, graphs[i].nnz);
+      S = si < 0 ? -1 : std::max(S, si);
+    }
+    if (S < 0) {
+      if (prec == MGL_PREC_EXACT)
+        FAIL(ctx, MGL_UNSUPPORTED,
+             "EXACT precision needs positive weights that are integer multiples of 2^-S");
+      prec = MGL_PREC_FP64;
+      S = 0;
+    }
+  }
+  if (prec == MGL_PREC_AUTO) prec = MGL_PREC_EXACT;
+  int mode0;
+  if (prec == MGL_PREC_EXACT) mode0 = weighted ? MODE_INT : MODE_UNIT;
+  else if (prec == MGL_PREC_FP64) mode0 = MODE_F64;
+  else if (prec == MGL_PREC_FP32_HASH) mode0 = MODE_F32;
+  else FAIL(ctx, MGL_INVALID_ARG, "unknown precision %d", o.precision);
+  scale = S;
+  // union sizes
+  vbase0.assign(G + 1, 0);
+  std::vector<int64_t> eb(G + 1, 0);
+  for (int i = 0; i < G; ++i) {
+    vbase0[i + 1] = vbase0[i] + graphs[i].n;
+    eb[i + 1] = eb[i] + graphs[i].nnz;
+  }
+  n0 = vbase0[G];
+  nnz0 = eb[G];
+  if (n0 > 0x7fffffffLL) FAIL(ctx, MGL_UNSUPPORTED, "batch has more than 2^31-1 vertices");
+  // per-graph 2m (host_graph.cpp:112-114)
+  hgs.assign(G, GState{});
+  for (int i = 0; i < G; ++i) {
+    const mgl_csr& gr = graphs[i];
+    GState& s = hgs[i];
+    if (prec == MGL_PREC_EXACT) {
+      if (!gr.w) {
+        s.mass_i = gr.nnz;
+      } else {
+        std::vector<int64_t> part(16, 0);
+        const int sc = S;
+        parallel_for(gr.nnz, [&](int64_t b, int64_t e, int t) {
+          int64_t acc = 0;
+          for (int64_t j = b; j < e; ++j) acc += (int64_t)std::ldexp((double)gr.w[j], sc);
+          part[t] = acc;
+        });
+        for (auto x : part) s.mass_i += x;
+      }
+      s.mass = (double)s.mass_i;
+    } else {
+      double m = 0;
+      if (!gr.w) m = (double)gr.nnz;
+      else
+        for (int64_t j = 0; j < gr.nnz; ++j) m += (double)gr.w[j];
+      s.mass = m;
+    }
+  }
+  // device union graph
+  LevelGraph& g = lg[0];
+  cur = 0;
+  g.mode = mode0;
+  g.n = n0;
+  g.nnz = nnz0;
+  g.voff = vbase0;
+  g.eoff = eb;
+  HIPC(ctx, g.off.ensure(8 * (n0 + 1)));
+  HIPC(ctx, g.col.ensure(4 * std::max<int64_t>(nnz0, 1)));
+  for (int i = 0; i < G; ++i) {
+    const mgl_csr& gr = graphs[i];
+    if (gr.n > 0) {
+      HIPC(ctx, hipMemcpyAsync(g.off.as<int64_t>() + vbase0[i], gr.offsets, 8 * gr.n,
+                               hipMemcpyHostToDevice, st()));
+      if (eb[i]) k_add_i64<<<grid_n(gr.n), 256, 0, st()>>>(g.off.as<int64_t>() + vbase0[i], gr.n, eb[i]);
+    }
+    if (gr.nnz > 0) {
+      HIPC(ctx, hipMemcpyAsync(g.col.as<int32_t>() + eb[i], gr.cols, 4 * gr.nnz,
+                               hipMemcpyHostToDevice, st()));
+      if (vbase0[i])
+        k_add_i32<<<grid_n(gr.nnz), 256, 0, st()>>>(g.col.as<int32_t>() + eb[i], gr.nnz,
+                                                   (int32_t)vbase0[i]);
+    }
+  }
+  HIPC(ctx, hipMemcpyAsync(g.off.as<int64_t>() + n0, &nnz0, 8, hipMemcpyHostToDevice, st()));
+  if (mode0 != MODE_UNIT) {
+    HIPC(ctx, g.w.ensure(wsize(mode0) * std::max<int64_t>(nnz0, 1)));
+    DBuf stage;
+    if (weighted) HIPC(ctx, stage.ensure(4 * std::max<int64_t>(nnz0, 1)));
+    for (int i = 0; i < G; ++i) {
+      const mgl_csr& gr = graphs[i];
+      if (gr.nnz == 0) continue;
+      if (gr.w) {
+        HIPC(ctx, hipMemcpyAsync(stage.as<float>() + eb[i], gr.w, 4 * gr.nnz,
+                                 hipMemcpyHostToDevice, st()));
+      } else if (mode0 == MODE_INT) {
+        // unit weights of an unweighted graph inside a weighted batch
+        std::vector<int64_t> one(gr.nnz, (int64_t)1 << S);
+        HIPC(ctx, hipMemcpyAsync(g.w.as<int64_t>() + eb[i], one.data(), 8 * gr.nnz,
+                                 hipMemcpyHostToDevice, st()));
+        HIPC(ctx, hipStreamSynchronize(st()));
+        continue;
+      } else {
+        k_fill_one<<<grid_n(gr.nnz), 256, 0, st()>>>(mode0, (char*)g.w.p + wsize(mode0) * eb[i], gr.nnz);
+        continue;
+      }
+      launch_cast_weights(mode0, stage.as<float>() + eb[i], (char*)g.w.p + wsize(mode0) * eb[i],
+                          gr.nnz, S, st());
+    }
+    HIPC(ctx, hipStreamSynchronize(st()));
+  }
+  if (G > 1) {
+    HIPC(ctx, g.gid.ensure(4 * std::max<int64_t>(n0, 1)));
+    HIPC(ctx, bgid0.ensure(4 * std::max<int64_t>(n0, 1)));
+    DBuf dv;
+    HIPC(ctx, dv.ensure(8 * (G + 1)));
+    HIPC(ctx, hipMemcpyAsync(dv.p, vbase0.data(), 8 * (G + 1), hipMemcpyHostToDevice, st()));
+    launch_gid_fill(dv.as<int64_t>(), G, g.gid.as<int32_t>(), st());
+    launch_copy_vid(g.gid.as<int32_t>(), bgid0.as<int32_t>(), n0, st());
+    HIPC(ctx, hipStreamSynchronize(st()));
+  }
+  mgl_status s = alloc_state();
+  if (s) return s;
+  HIPC(ctx, hipMemcpyAsync(bgs.p, hgs.data(), sizeof(GState) * G, hipMemcpyHostToDevice, st()));
+  HIPC(ctx, hipStreamSynchronize(st()));
+  HIPC(ctx, hipGetLastError());
+  return MGL_OK;
+}
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+void mgl_opts_default(mgl_opts* o) {
+  if (!o) return;
+  o->version = 2;
+  o->threshold = 1.0e-6;
+  o->max_phases = 1000;
+  o->max_iter = 0;
+  o->precision = MGL_PREC_AUTO;
+  o->tie_rule = MGL_TIE_REFERENCE;
+  o->fuse_modularity = 1;
+  o->keep_levels = 0;
+  o->keep_graphs = 0;
+  o->validate = 1;
+  o->tuning_file = nullptr;
+}
+
+int mgl_version_major(void) { return 1; }
+
+static mgl_status load_tuning(mgl_ctx* ctx, const char* path);
+
+mgl_status mgl_create(mgl_ctx** out, int device) {
+  if (!out) return MGL_INVALID_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+    (void)hipGetLastError();
+    return MGL_DEVICE_ERROR;
+  }
+  if (device < 0 || device >= ndev) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = new mgl_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->st) != hipSuccess) {
+    (void)hipGetLastError();
+    delete ctx;
+    return MGL_DEVICE_ERROR;
+  }
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
+          hipSuccess &&
+      lds > 0)
+    ctx->lds_max = lds;
+  (void)hipGetLastError();
+  move_kernels_allow_lds(ctx->lds_max);
+  (void)hipGetLastError();
+  *out = ctx;
+  return MGL_OK;
+}
+
+void mgl_destroy(mgl_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->st) (void)hipStreamDestroy(ctx->st);
+  delete ctx;
+}
+
+const char* mgl_last_error(const mgl_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+static mgl_status check_opts(mgl_ctx* ctx, const mgl_opts* o) {
+  if (o->version < 0 || o->version > 2) FAIL(ctx, MGL_INVALID_ARG, "version must be 0, 1 or 2");
+  if (!(o->threshold >= 0) || !std::isfinite(o->threshold))
+    FAIL(ctx, MGL_INVALID_ARG, "threshold must be finite and >= 0");
+  if (o->max_phases < 0) FAIL(ctx, MGL_INVALID_ARG, "max_phases must be >= 0");
+  if (o->tie_rule != MGL_TIE_REFERENCE && o->tie_rule != MGL_TIE_SMALLEST)
+    FAIL(ctx, MGL_INVALID_ARG, "unknown tie rule");
+  return MGL_OK;
+}
+
+mgl_status mgl_louvain_batched(mgl_ctx* ctx, const mgl_csr* graphs, int32_t n_graphs,
+                               const mgl_opts* opts, mgl_result** results) {
+  if (!ctx) return MGL_INVALID_ARG;
+  if (!graphs || !results || n_graphs < 1) FAIL(ctx, MGL_INVALID_ARG, "null argument");
+  HIPC(ctx, hipSetDevice(ctx->device));
+  mgl_opts o;
+  if (opts) o = *opts;
+  else mgl_opts_default(&o);
+  mgl_status s = check_opts(ctx, &o);
+  if (s) return s;
+  if (o.tuning_file) {
+    s = load_tuning(ctx, o.tuning_file);
+    if (s) return s;
+  }
+  for (int i = 0; i < n_graphs; ++i) results[i] = nullptr;
+  Engine E(ctx);
+  E.o = o;
+  s = E.setup_union(graphs, n_graphs);
+  if (s) return s;
+  s = E.alloc_coarsen();
+  if (s) return s;
+  std::vector<mgl_result*> out(n_graphs);
+  for (int i = 0; i < n_graphs; ++i) out[i] = new mgl_result();
+  E.res = out;
+  s = E.run(out);
+  if (s) {
+    for (auto* r : out) delete r;
+    return s;
+  }
+  for (int i = 0; i < n_graphs; ++i) results[i] = out[i];
+  return MGL_OK;
+}
+
+mgl_status mgl_louvain(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* o, mgl_result** out) {
+  if (!ctx) return MGL_INVALID_ARG;
+  if (!g || !out) FAIL(ctx, MGL_INVALID_ARG, "null argument");
+  return mgl_louvain_batched(ctx, g, 1, o, out);
+}
+
+int mgl_result_levels(const mgl_result* r) { return r ? (int)r->levels.size() : 0; }
+int64_t mgl_result_level_n(const mgl_result* r, int l) {
+  return (r && l >= 0 && l < (int)r->levels.size()) ? r->levels[l].n : -1;
+}
+double mgl_result_level_q(const mgl_result* r, int l) {
+  return (r && l >= 0 && l < (int)r->levels.size()) ? r->levels[l].q : NAN;
+}
+int mgl_result_level_iters(const mgl_result* r, int l) {
+  return (r && l >= 0 && l < (int)r->levels.size()) ? r->levels[l].iters : -1;
+}
+const double* mgl_result_level_qtraj(const mgl_result* r, int l) {
+  return (r && l >= 0 && l < (int)r->levels.size()) ? r->levels[l].qtraj.data() : nullptr;
+}
+const int32_t* mgl_result_level_assign(const mgl_result* r, int l) {
+  if (!r || l < 0 || l >= (int)r->levels.size() || r->levels[l].assign.size() != (size_t)r->levels[l].n)
+    return nullptr;
+  return r->levels[l].assign.data();
+}
+mgl_status mgl_result_level_graph(const mgl_result* r, int l, int64_t* n, int64_t* nnz,
+                                  const int64_t** off, const int32_t** col, const double** w) {
+  if (!r || l < 1 || l >= (int)r->levels.size() || !r->levels[l].has_graph)
+    return MGL_INVALID_ARG;
+  const LevelOut& lo = r->levels[l];
+  if (n) *n = lo.gn;
+  if (nnz) *nnz = lo.gnnz;
+  if (off) *off = lo.goff.data();
+  if (col) *col = lo.gcol.data();
+  if (w) *w = lo.gw.data();
+  return MGL_OK;
+}
+const int32_t* mgl_result_final_assign(const mgl_result* r) {
+  return r ? r->final_assign.data() : nullptr;
+}
+double mgl_result_final_q(const mgl_result* r) { return r ? r->final_q : NAN; }
+int mgl_result_precision(const mgl_result* r) { return r ? r->precision : -1; }
+double mgl_result_seconds(const mgl_result* r) { return r ? r->seconds : NAN; }
+void mgl_result_free(mgl_result* r) { delete r; }
+
+int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins) {
+  if (!ctx) return 0;
+  BinLayout L = layout_for(ctx, MODE_UNIT);
+  for (int i = 0; i < L.nb && i < max_bins; ++i) upper[i] = L.b[i].upper;
+  return L.nb;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Sessions: one graph resident in HBM, driven pass by pass at level 0.
+struct mgl_session {
+  mgl_ctx* ctx = nullptr;
+  Engine* E = nullptr;
+  int it = 0;
+  bool coarsen_ready = false;
+  mgl_pass_stats st{};
+  std::vector<int64_t> hoff;  // host copy of the level-0 offsets (statistics)
+};
+
+__global__ void k_reactivate(GState* gs) {
+  GState& s = gs[0];
+  s.active = 1;
+  s.fix = 0;
+  s.sin_i = 0; s.sin_d = 0; s.sa2_lo = 0; s.sa2_hi = 0; s.sa2_d = 0;
+}
+__global__ void k_zero_stats(GState* gs, unsigned long long* dist) {
+  gs[0].moved = 0;
+  *dist = 0;
+}
+
+static mgl_status session_prime(mgl_session* s) {
+  // fused mode needs the first proposal before the first decided pass
+  Engine& E = *s->E;
+  if (!E.o.fuse_modularity) return MGL_OK;
+  const int up = ((s->it + 1) & 1) ? 1 : 0;
+  mgl_status r = E.move(E.c, E.p, up, nullptr, false);
+  if (r) return r;
+  LevelGraph& g = E.LG();
+  launch_update(g.mode, g.n, E.c, E.p, E.bk.p, E.ba.p, nullptr, E.bgs.as<GState>(), E.st());
+  launch_reduce(g.mode, g.n, nullptr, E.ba.p, nullptr, E.bgs.as<GState>(), E.st());
+  launch_decide(1, E.bgs.as<GState>(), E.o.threshold, E.o.max_iter, 1, mode_exact(g.mode),
+                nullptr, 0, 1, E.st());
+  HIPC(s->ctx, hipGetLastError());
+  return MGL_OK;
+}
+
+extern "C" {
+
+mgl_status mgl_session_create(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* opts,
+                              mgl_session** out) {
+  if (!ctx) return MGL_INVALID_ARG;
+  if (!g || !out) FAIL(ctx, MGL_INVALID_ARG, "null argument");
+  HIPC(ctx, hipSetDevice(ctx->device));
+  *out = nullptr;
+  mgl_opts o;
+  if (opts) o = *opts;
+  else mgl_opts_default(&o);
+  mgl_status s = check_opts(ctx, &o);
+  if (s) return s;
+  if (o.tuning_file) {
+    s = load_tuning(ctx, o.tuning_file);
+    if (s) return s;
+  }
+  mgl_session* ss = new mgl_session();
+  ss->ctx = ctx;
+  ss->E = new Engine(ctx);
+  ss->E->o = o;
+  s = ss->E->setup_union(g, 1);
+  if (s) { delete ss->E; delete ss; return s; }
+  ss->E->running.assign(1, 1);
+  ss->E->Q_host.assign(1, 0.0);
+  ss->E->Q_old.assign(1, -1.0);
+  ss->E->phase.assign(1, 0);
+  s = ss->E->level_start();
+  if (s) { delete ss->E; delete ss; return s; }
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  *out = ss;
+  return MGL_OK;
+}
+
+void mgl_session_destroy(mgl_session* s) {
+  if (!s) return;
+  (void)hipSetDevice(s->ctx->device);
+  delete s->E;
+  delete s;
+}
+
+mgl_status mgl_session_reset(mgl_session* s, double* q0) {
+  if (!s) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  mgl_status r = E.level_start();
+  if (r) return r;
+  s->it = 0;
+  r = E.read_gs();
+  if (r) return r;
+  if (q0) *q0 = E.hgs[0].q_cur;
+  r = session_prime(s);
+  if (r) return r;
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  return MGL_OK;
+}
+
+mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
+  if (!s) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  unsigned long long* dist = E.bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS;
+  k_zero_stats<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>(), dist);
+  E.timing = true;
+  s->it += 1;
+  mgl_status r = E.iteration(s->it);
+  E.timing = false;
+  if (r) return r;
+  r = E.read_gs();
+  if (r) return r;
+  const GState& gs = E.hgs[0];
+  const bool acc = E.o.fuse_modularity ? (gs.fix == 0 || gs.fix == 2) && gs.fix != 1
+                                       : (gs.fix == 3);
+  if (accepted) *accepted = acc ? 1 : 0;
+  if (q) *q = gs.q_cur;
+  // statistics of the pass
+  mgl_pass_stats& st = s->st;
+  memset(&st, 0, sizeof(st));
+  st.n = E.LG().n;
+  st.nnz = E.LG().nnz;
+  float ms = 0;
+  hipEvent_t* ev = E.ev;
+  const int B = 2 * MGL_MAX_BINS;
+  HIPC(ctx, hipEventElapsedTime(&ms, ev[B + 0], ev[B + 1]));
+  st.move_ms = ms;
+  HIPC(ctx, hipEventElapsedTime(&ms, ev[B + 1], ev[B + 2]));
+  st.update_ms = ms;
+  HIPC(ctx, hipEventElapsedTime(&ms, ev[B + 2], ev[B + 3]));
+  st.modularity_ms = ms;
+  HIPC(ctx, hipEventElapsedTime(&ms, ev[B + 0], ev[B + 4]));
+  st.pass_ms = ms;
+  st.dominant_ms = -1;
+  std::vector<int64_t> deg_sum;
+  for (int b = E.L.nb - 1; b >= 1; --b) {
+    if (E.bin_n[b] == 0) continue;
+    HIPC(ctx, hipEventElapsedTime(&ms, ev[2 * b], ev[2 * b + 1]));
+    if (ms > st.dominant_ms) {
+      st.dominant_ms = ms;
+      st.dominant_bin = b;
+    }
+  }
+  if (st.dominant_ms >= 0) {
+    const int b = st.dominant_bin;
+    st.dominant_vertices = E.bin_n[b];
+    // CSR entries of the dominant bin's vertices
+    std::vector<int32_t> perm(E.bin_n[b]);
+    HIPC(ctx, hipMemcpy(perm.data(), E.bperm.as<vid>() + E.bin_off[b], 4 * E.bin_n[b],
+                        hipMemcpyDeviceToHost));
+    std::vector<int64_t>& hoff = s->hoff;
+    const LevelGraph& g = E.LG();
+    if ((int64_t)hoff.size() != g.n + 1) {
+      hoff.resize(g.n + 1);
+      HIPC(ctx, hipMemcpy(hoff.data(), g.off.p, 8 * (g.n + 1), hipMemcpyDeviceToHost));
+    }
+    int64_t ents = 0;
+    for (int32_t v : perm) ents += hoff[v + 1] - hoff[v];
+    st.dominant_entries = ents;
+  }
+  unsigned long long dh = 0;
+  HIPC(ctx, hipMemcpy(&dh, dist, 8, hipMemcpyDeviceToHost));
+  st.distinct = (int64_t)dh;
+  st.moved = gs.moved;
+  st.q = gs.q_cur;
+  // a rejected proposal ends the level; a session keeps passing, so it
+  // re-activates with community totals rebuilt from the committed assignment
+  if (!gs.active) {
+    k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
+    r = E.recompute_a(E.c);
+    if (r) return r;
+    r = session_prime(s);
+    if (r) return r;
+    HIPC(ctx, hipStreamSynchronize(ctx->st));
+  }
+  return MGL_OK;
+}
+
+mgl_status mgl_session_stats(const mgl_session* s, mgl_pass_stats* st) {
+  if (!s || !st) return MGL_INVALID_ARG;
+  *st = s->st;
+  return MGL_OK;
+}
+
+mgl_status mgl_session_bins(mgl_session* s, int64_t* sizes, int* nb) {
+  if (!s || !sizes || !nb) return MGL_INVALID_ARG;
+  Engine& E = *s->E;
+  const int cap = *nb;
+  *nb = E.L.nb;
+  for (int i = 0; i < E.L.nb && i < cap; ++i) sizes[i] = E.bin_n[i];
+  return MGL_OK;
+}
+
+mgl_status mgl_session_set_assign(mgl_session* s, const int32_t* c) {
+  if (!s || !c) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  const int64_t n = E.LG().n;
+  for (int64_t v = 0; v < n; ++v)
+    if (c[v] < 0 || c[v] >= n) FAIL(ctx, MGL_INVALID_ARG, "community id out of range");
+  HIPC(ctx, hipMemcpyAsync(E.c, c, 4 * n, hipMemcpyHostToDevice, ctx->st));
+  launch_copy_vid(E.c, E.p, n, ctx->st);
+  launch_copy_vid(E.c, E.nx, n, ctx->st);
+  k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
+  mgl_status r = E.recompute_a(E.c);
+  if (r) return r;
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  return MGL_OK;
+}
+
+mgl_status mgl_session_move(mgl_session* s, int up, int32_t* c_new) {
+  if (!s || !c_new) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  const int64_t n = E.LG().n;
+  k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
+  launch_copy_vid(E.c, E.p, n, ctx->st);  // isolated vertices keep their community
+  mgl_status r = E.move(E.c, E.p, up ? 1 : 0, nullptr, false);
+  if (r) return r;
+  HIPC(ctx, hipMemcpyAsync(c_new, E.p, 4 * n, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  return MGL_OK;
+}
+
+mgl_status mgl_session_modularity(mgl_session* s, double* q) {
+  if (!s || !q) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  LevelGraph& g = E.LG();
+  k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
+  launch_fill_bytes0(E.bvin.p, 8 * (size_t)std::max<int64_t>(g.n, 1), ctx->st);
+  mgl_status r = E.rows(OP_INTRA, E.c, nullptr, E.bvin.p, true);
+  if (r) return r;
+  launch_reduce(g.mode, g.n, E.bvin.p, E.ba.p, nullptr, E.bgs.as<GState>(), ctx->st);
+  r = E.read_gs();
+  if (r) return r;
+  const GState& st = E.hgs[0];
+  const double M = st.mass;
+  if (M == 0) *q = 0;
+  else if (mode_exact(g.mode))
+    *q = (double)st.sin_i / M - u128_to_double(st.sa2_hi, st.sa2_lo) / (M * M);
+  else
+    *q = st.sin_d / M - st.sa2_d / (M * M);
+  k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  return MGL_OK;
+}
+
+mgl_status mgl_session_get_weights(mgl_session* s, double* a, double* k) {
+  if (!s) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  const int64_t n = E.LG().n;
+  const bool ex = mode_exact(E.LG().mode);
+  std::vector<int64_t> bi(n);
+  std::vector<double> bd(n);
+  for (int which = 0; which < 2; ++which) {
+    double* dst = which == 0 ? a : k;
+    if (!dst) continue;
+    const void* src = which == 0 ? E.ba.p : E.bk.p;
+    if (ex) {
+      HIPC(ctx, hipMemcpy(bi.data(), src, 8 * n, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < n; ++i) dst[i] = std::ldexp((double)bi[i], -E.scale);
+    } else {
+      HIPC(ctx, hipMemcpy(dst, src, 8 * n, hipMemcpyDeviceToHost));
+    }
+  }
+  return MGL_OK;
+}
+
+mgl_status mgl_session_apply(mgl_session* s, const int32_t* c_new) {
+  if (!s || !c_new) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  LevelGraph& g = E.LG();
+  const int64_t n = g.n;
+  for (int64_t v = 0; v < n; ++v)
+    if (c_new[v] < 0 || c_new[v] >= n) FAIL(ctx, MGL_INVALID_ARG, "community id out of range");
+  k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
+  HIPC(ctx, hipMemcpyAsync(E.p, c_new, 4 * n, hipMemcpyHostToDevice, ctx->st));
+  launch_update(g.mode, n, E.c, E.p, E.bk.p, E.ba.p, nullptr, E.bgs.as<GState>(), ctx->st);
+  launch_copy_vid(E.p, E.c, n, ctx->st);
+  launch_copy_vid(E.p, E.nx, n, ctx->st);
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  HIPC(ctx, hipGetLastError());
+  return MGL_OK;
+}
+
+mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64_t* off2,
+                               int32_t* col2, double* w2, int32_t* rank) {
+  if (!s || !n2 || !nnz2) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  if (!s->coarsen_ready) {
+    mgl_status r = E.alloc_coarsen();
+    if (r) return r;
+    s->coarsen_ready = true;
+  }
+  LevelGraph& g = E.LG();
+  const int out_mode = coarse_mode(g.mode);
+  DBuf doff, dcol, dw;
+  HIPC(ctx, doff.ensure(8 * (g.n + 2)));
+  HIPC(ctx, dcol.ensure(4 * std::max<int64_t>(g.nnz, 1)));
+  HIPC(ctx, dw.ensure(8 * std::max<int64_t>(g.nnz, 1)));
+  if (g.mode != MODE_UNIT) {
+    HIPC(ctx, E.cv0.ensure(8 * std::max<int64_t>(g.nnz, 1)));
+    HIPC(ctx, E.cv1.ensure(8 * std::max<int64_t>(g.nnz, 1)));
+    E.CB.v0 = E.cv0.p;
+    E.CB.v1 = E.cv1.p;
+  }
+  int32_t one = 1;
+  HIPC(ctx, hipMemcpyAsync(E.bkeep.p, &one, 4, hipMemcpyHostToDevice, ctx->st));
+  int64_t C = 0, E2 = 0;
+  const int rc = coarsen_level(g.mode, out_mode, g.n, g.off.as<eid>(), g.col.as<vid>(), g.w.p,
+                               E.c, nullptr, E.bkeep.as<int32_t>(), nullptr, g.nnz, E.CB,
+                               doff.as<eid>(), dcol.as<vid>(), dw.p, &C, &E2, ctx->st);
+  if (rc) FAIL(ctx, MGL_DEVICE_ERROR, "coarsening failed (%d)", rc);
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  *n2 = C;
+  *nnz2 = E2;
+  if (off2) HIPC(ctx, hipMemcpy(off2, doff.p, 8 * (C + 1), hipMemcpyDeviceToHost));
+  if (col2 && E2) HIPC(ctx, hipMemcpy(col2, dcol.p, 4 * E2, hipMemcpyDeviceToHost));
+  if (w2 && E2) {
+    if (out_mode == MODE_INT) {
+      std::vector<int64_t> t(E2);
+      HIPC(ctx, hipMemcpy(t.data(), dw.p, 8 * E2, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < E2; ++i) w2[i] = std::ldexp((double)t[i], -E.scale);
+    } else if (out_mode == MODE_F64) {
+      HIPC(ctx, hipMemcpy(w2, dw.p, 8 * E2, hipMemcpyDeviceToHost));
+    } else {
+      std::vector<float> t(E2);
+      HIPC(ctx, hipMemcpy(t.data(), dw.p, 4 * E2, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < E2; ++i) w2[i] = t[i];
+    }
+  }
+  if (rank && g.n) {
+    std::vector<int64_t> rk(g.n + 1);
+    std::vector<int32_t> fl(g.n);
+    HIPC(ctx, hipMemcpy(rk.data(), E.CB.rank, 8 * (g.n + 1), hipMemcpyDeviceToHost));
+    HIPC(ctx, hipMemcpy(fl.data(), E.CB.flag, 4 * g.n, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < g.n; ++i) rank[i] = fl[i] ? (int32_t)rk[i] : -1;
+  }
+  return MGL_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Tuning file: one line per bin, "upper shape s lg" (text). Written by
+// mgl_autotune, read at context use.
+static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
+  FILE* f = fopen(path, "r");
+  if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot open tuning file %s", path);
+  BinLayout L;
+  L.nb = 0;
+  char line[256];
+  while (fgets(line, sizeof(line), f) && L.nb < MGL_MAX_BINS) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    long long up;
+    int sh, s, lg;
+    if (sscanf(line, "%lld %d %d %d", &up, &sh, &s, &lg) == 4) {
+      L.b[L.nb++] = {up < 0 ? INT64_MAX : (int64_t)up, sh, s, lg};
+    }
+  }
+  fclose(f);
+  if (L.nb < 2 || L.b[0].upper != 0 || L.b[0].shape != SHAPE_NONE)
+    FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin 0 must be degree 0", path);
+  for (int i = 1; i < L.nb; ++i) {
+    const BinSpec& b = L.b[i];
+    if (b.upper <= L.b[i - 1].upper) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bounds", path);
+    if (b.shape == SHAPE_GROUP || b.shape == SHAPE_BLOCK) {
+      // a table of 2^lg slots must hold every distinct community of a row
+      if (b.lg < 1 || b.lg > 15 || b.upper > ((int64_t)1 << b.lg))
+        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table too small", path, i);
+    } else if (b.shape != SHAPE_GLOBAL) {
+      FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d shape", path, i);
+    }
+  }
+  if (L.b[L.nb - 1].shape != SHAPE_GLOBAL)
+    FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: last bin must use the HBM hash", path);
+  ctx->user_bins = L;
+  ctx->have_user_bins = true;
+  return MGL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuner: times the level-0 move pass of g under candidate bin layouts
+// (degree bounds, lanes per vertex, LDS table sizes, workgroup sizes) and
+// writes the fastest as a tuning file.
+static std::vector<BinLayout> tuning_candidates() {
+  std::vector<BinLayout> out;
+  out.push_back(default_bins());
+  auto mk = [](std::initializer_list<BinSpec> l) {
+    BinLayout L;
+    L.nb = 0;
+    for (const auto& b : l) L.b[L.nb++] = b;
+    return L;
+  };
+  // two entries per lane in the sub-wave groups
+  out.push_back(mk({{0, SHAPE_NONE, 0, 0}, {8, SHAPE_GROUP, 4, 4}, {16, SHAPE_GROUP, 8, 5},
+                    {32, SHAPE_GROUP, 16, 6}, {64, SHAPE_GROUP, 32, 7}, {128, SHAPE_GROUP, 64, 8},
+                    {256, SHAPE_GROUP, 64, 9}, {512, SHAPE_GROUP, 64, 10},
+                    {2048, SHAPE_BLOCK, 512, 12}, {4096, SHAPE_BLOCK, 1024, 13},
+                    {INT64_MAX, SHAPE_GLOBAL, 1024, 0}}));
+  // wavefront per vertex from degree 33, larger workgroups earlier
+  out.push_back(mk({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_GROUP, 4, 3}, {8, SHAPE_GROUP, 8, 4},
+                    {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6}, {64, SHAPE_GROUP, 64, 7},
+                    {128, SHAPE_GROUP, 64, 8}, {512, SHAPE_BLOCK, 256, 10},
+                    {2048, SHAPE_BLOCK, 512, 12}, {4096, SHAPE_BLOCK, 1024, 13},
+                    {INT64_MAX, SHAPE_GLOBAL, 1024, 0}}));
+  // denser LDS tables (load factor up to 1)
+  out.push_back(mk({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_GROUP, 4, 3}, {8, SHAPE_GROUP, 8, 4},
+                    {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6}, {64, SHAPE_GROUP, 64, 7},
+                    {128, SHAPE_GROUP, 64, 8}, {512, SHAPE_GROUP, 64, 9},
+                    {2048, SHAPE_BLOCK, 256, 11}, {8192, SHAPE_BLOCK, 1024, 13},
+                    {INT64_MAX, SHAPE_GLOBAL, 1024, 0}}));
+  return out;
+}
+
+extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* path) {
+  if (!ctx) return MGL_INVALID_ARG;
+  if (!g || !path) FAIL(ctx, MGL_INVALID_ARG, "null argument");
+  const BinLayout saved = ctx->user_bins;
+  const bool had = ctx->have_user_bins;
+  double best_ms = 1e300;
+  BinLayout best = default_bins();
+  for (const BinLayout& cand : tuning_candidates()) {
+    ctx->user_bins = cand;
+    ctx->have_user_bins = true;
+    mgl_opts o;
+    mgl_opts_default(&o);
+    mgl_session* s = nullptr;
+    mgl_status st = mgl_session_create(ctx, g, &o, &s);
+    if (st) { ctx->user_bins = saved; ctx->have_user_bins = had; return st; }
+    double q;
+    int acc;
+    std::vector<double> t;
+    st = mgl_session_reset(s, &q);
+    for (int i = 0; i < 4 && st == MGL_OK; ++i) {
+      st = mgl_session_pass(s, &q, &acc);
+      if (st == MGL_OK && i > 0) t.push_back(s->st.move_ms);
+    }
+    mgl_session_destroy(s);
+    if (st) { ctx->user_bins = saved; ctx->have_user_bins = had; return st; }
+    std::sort(t.begin(), t.end());
+    const double med = t.empty() ? 1e300 : t[t.size() / 2];
+    if (med < best_ms) { best_ms = med; best = cand; }
+  }
+  ctx->user_bins = saved;
+  ctx->have_user_bins = had;
+  FILE* f = fopen(path, "w");
+  if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot write %s", path);
+  fprintf(f, "# mgl bin layout: upper shape lanes_or_threads log2_slots (move median %.4f ms)\n",
+          best_ms);
+  for (int i = 0; i < best.nb; ++i)
+    fprintf(f, "%lld %d %d %d\n", best.b[i].upper == INT64_MAX ? -1LL : (long long)best.b[i].upper,
+            best.b[i].shape, best.b[i].s, best.b[i].lg);
+  fclose(f);
+  return MGL_OK;
+}

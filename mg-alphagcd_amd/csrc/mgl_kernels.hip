@@ -1,0 +1,854 @@
+// gfx950 kernels of the Louvain pass: degree binning, binned modularity-gain
+// moves, per-vertex row reductions, community-weight update, per-graph
+// modularity reduction and the device-side inner-loop control.
+//
+// Reference behaviour (NikodemGapski/MG-alphaGCD):
+//   move rule          src/louvain/louvain.cu:454-585, :633-805, :855-1026
+//   v1 filter          src/louvain/louvain_gl.cu:504,578
+//   vertex weights     src/louvain/louvain.cu:30-74
+//   weight update      src/louvain/louvain.cu:1299-1396; louvain_baseline.cu:1319-1372
+//   modularity         src/louvain/louvain.cu:105-238
+//   binning            src/bin/BIN.cu:33-121, 298-342, 468-499
+//   inner loop control src/louvain/louvain.cu:1548-1660
+//
+// The gain is evaluated in FP64 in the reference's exact expression order;
+// this file is compiled with -ffp-contract=off and the pragma below so that no
+// multiply-add is fused, which keeps the gains bit-identical to the oracle's.
+#pragma clang fp contract(off)
+
+#include "mgl_kernels.h"
+#include <climits>
+
+// ---------------------------------------------------------------------------
+// atomics on any slot type
+__device__ __forceinline__ void atom_add(int32_t* p, int32_t x) { atomicAdd(p, x); }
+__device__ __forceinline__ void atom_add(int64_t* p, int64_t x) {
+  atomicAdd((unsigned long long*)p, (unsigned long long)x);
+}
+__device__ __forceinline__ void atom_add(float* p, float x) { atomicAdd(p, x); }
+__device__ __forceinline__ void atom_add(double* p, double x) { atomicAdd(p, x); }
+
+// Insert (key, w) into an LDS open-addressing table of 2^lg slots. Keys are
+// written once (EMPTY -> key) and never change afterwards, so a plain read that
+// sees a key is final; a stale EMPTY only costs a CAS that returns the key.
+template <class ST>
+__device__ __forceinline__ void lds_insert(vid* K, ST* V, vid key, ST w, int lg) {
+  const uint32_t mask = (1u << lg) - 1u;
+  uint32_t h = slot_hash(key, lg);
+  while (true) {
+    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == MGL_EMPTY) cur = atomicCAS(&K[h], MGL_EMPTY, key);
+    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
+    h = (h + 1u) & mask;
+  }
+}
+
+// Same for a table in HBM; reads are agent-coherent loads (no stale L1 lines).
+template <class ST>
+__device__ __forceinline__ void glb_insert(vid* K, ST* V, vid key, ST w, int lg) {
+  const uint32_t mask = (1u << lg) - 1u;
+  uint32_t h = slot_hash(key, lg);
+  while (true) {
+    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cur == MGL_EMPTY) cur = atomicCAS(&K[h], MGL_EMPTY, key);
+    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
+    h = (h + 1u) & mask;
+  }
+}
+
+template <class M>
+__device__ __forceinline__ bool vertex_active(const MoveArgs& A, vid v, int& g) {
+  g = A.gid ? A.gid[v] : 0;
+  return A.gs[g].active != 0;
+}
+
+// Per-vertex gain of moving into cj (louvain.cu:540-543, same operation order):
+//   wij = e_{v,cj}; wij -= (eici - ki*(aci - acj)/(2*M)); wij /= M;  M = mass/2
+__device__ __forceinline__ double gain(double e, double eici, double ki, double aci,
+                                       double acj, double Mh) {
+  double wij = e;
+  wij -= (eici - ki * (aci - acj) / (2. * Mh));
+  wij /= Mh;
+  return wij;
+}
+
+__device__ __forceinline__ vid clamp_dst(vid dst, vid ci, int up, int version) {
+  if (version == 1) return dst;                      // louvain_gl.cu:578
+  return up ? (dst < ci ? dst : ci) : (dst > ci ? dst : ci);  // louvain.cu:579-585
+}
+
+// ---------------------------------------------------------------------------
+// SHAPE_GROUP: S lanes per vertex, 256/S vertices per workgroup, 2^LG LDS
+// slots per vertex.
+template <class M, int S, int LG>
+__global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __restrict__ perm,
+                                                    int64_t nb) {
+  typedef typename M::S ST;
+  typedef typename M::K KT;
+  constexpr int GPB = 256 / S;
+  constexpr int H = 1 << LG;
+  __shared__ vid sk[GPB * H];
+  __shared__ ST sv[GPB * H];
+  __shared__ unsigned long long sdist;
+  const int grp = threadIdx.x / S, lane = threadIdx.x & (S - 1);
+  const int64_t gi = (int64_t)blockIdx.x * GPB + grp;
+  vid* K = sk + grp * H;
+  ST* V = sv + grp * H;
+#pragma unroll
+  for (int i = lane; i < H; i += S) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  if (threadIdx.x == 0) sdist = 0;
+  vid v = 0;
+  int g = 0;
+  bool ok = gi < nb;
+  if (ok) { v = perm[gi]; ok = vertex_active<M>(A, v, g); }
+  __syncthreads();
+  ST eici = 0, selfw = 0;
+  vid ci = 0;
+  if (ok) {
+    ci = A.c[v];
+    const eid e0 = A.off[v], e1 = A.off[v + 1];
+    const bool v1 = A.version == 1;
+    for (eid e = e0 + lane; e < e1; e += S) {
+      const vid u = A.col[e];
+      const vid cj = A.c[u];
+      const ST wv = wload<M>(A.w, e);
+      const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
+      if (cand) lds_insert<ST>(K, V, cj, wv, LG);
+      else if (cj == ci) { if (u != v) eici += wv; else selfw += wv; }
+    }
+  }
+  __syncthreads();
+  if (ok) {
+    eici = group_sum<S>(eici);
+    selfw = group_sum<S>(selfw);
+    const double Mh = A.gs[g].mass / 2.0;
+    const KT ki = ((const KT*)A.k)[v];
+    const KT* a = (const KT*)A.a;
+    const KT aci_k = a[ci] - ki;
+    const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
+    const bool small = A.tie_small ? true : (A.up != 0);
+    double best = 0.0;
+    vid dst = ci;
+    int cnt = 0;
+#pragma unroll
+    for (int i = lane; i < H; i += S) {
+      const vid cj = K[i];
+      if (cj != MGL_EMPTY) {
+        ++cnt;
+        const double wij = gain((double)V[i], deici, dki, daci, (double)a[cj], Mh);
+        if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+      }
+    }
+    group_argmax<S>(best, dst, small);
+    if (A.stats) {
+      cnt = group_sum<S>(cnt);
+      if (lane == 0) atomicAdd(&sdist, (unsigned long long)cnt);
+    }
+    if (lane == 0) {
+      A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
+      if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
+    }
+  }
+  if (A.stats) {
+    __syncthreads();
+    if (threadIdx.x == 0 && sdist) atomicAdd(A.stats, sdist);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Workgroup-wide helpers (BS threads, wave64).
+template <int BS, class T>
+__device__ __forceinline__ T block_sum(T v, T* red /* [BS/64] */) {
+  v = group_sum<64>(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < BS / 64; ++i) s += red[i];
+  return s;
+}
+
+template <int BS>
+__device__ __forceinline__ void block_argmax(double& bg, vid& bd, bool small, double* rg,
+                                             vid* rd) {
+  group_argmax<64>(bg, bd, small);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) { rg[w] = bg; rd[w] = bd; }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < BS / 64; ++i)
+    if (better(rg[i], rd[i], bg, bd, small)) { bg = rg[i]; bd = rd[i]; }
+}
+
+// SHAPE_BLOCK / SHAPE_GLOBAL: one workgroup per vertex. GLOBAL keeps the
+// table in HBM (keys gk, values gv at toff[i]); BLOCK in dynamic LDS.
+template <class M, int BS, bool GLOBAL>
+__global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __restrict__ perm,
+                                                   int64_t nb, int lg_lds,
+                                                   const int64_t* __restrict__ toff, vid* gk,
+                                                   typename M::S* gv) {
+  typedef typename M::S ST;
+  typedef typename M::K KT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double rg[BS / 64];
+  __shared__ vid rd[BS / 64];
+  __shared__ ST rs[BS / 64];
+  __shared__ int rc[BS / 64];
+  const int64_t gi = blockIdx.x;
+  if (gi >= nb) return;
+  const vid v = perm[gi];
+  int g;
+  if (!vertex_active<M>(A, v, g)) return;  // uniform over the workgroup
+  int lg;
+  vid* K;
+  ST* V;
+  if (GLOBAL) {
+    const int64_t H = toff[gi + 1] - toff[gi];
+    lg = 63 - __builtin_clzll((unsigned long long)H);
+    K = gk + toff[gi];
+    V = gv + toff[gi];
+  } else {
+    lg = lg_lds;
+    K = (vid*)smem;
+    V = (ST*)(smem + (((size_t)4 << lg) + 15) / 16 * 16);
+  }
+  const int H = 1 << lg;
+  for (int i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  __syncthreads();
+  const vid ci = A.c[v];
+  const eid e0 = A.off[v], e1 = A.off[v + 1];
+  const bool v1 = A.version == 1;
+  ST eici = 0, selfw = 0;
+  for (eid e = e0 + threadIdx.x; e < e1; e += BS) {
+    const vid u = A.col[e];
+    const vid cj = A.c[u];
+    const ST wv = wload<M>(A.w, e);
+    const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
+    if (cand) {
+      if (GLOBAL) glb_insert<ST>(K, V, cj, wv, lg);
+      else lds_insert<ST>(K, V, cj, wv, lg);
+    } else if (cj == ci) {
+      if (u != v) eici += wv; else selfw += wv;
+    }
+  }
+  if (GLOBAL) __threadfence();
+  __syncthreads();
+  eici = block_sum<BS>(eici, rs);
+  selfw = block_sum<BS>(selfw, rs);
+  const double Mh = A.gs[g].mass / 2.0;
+  const KT ki = ((const KT*)A.k)[v];
+  const KT* a = (const KT*)A.a;
+  const KT aci_k = a[ci] - ki;
+  const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
+  const bool small = A.tie_small ? true : (A.up != 0);
+  double best = 0.0;
+  vid dst = ci;
+  int cnt = 0;
+  for (int i = threadIdx.x; i < H; i += BS) {
+    vid cj;
+    ST ev;
+    if (GLOBAL) {
+      cj = __hip_atomic_load(&K[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      ev = (cj != MGL_EMPTY) ? __hip_atomic_load(&V[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (ST)0;
+    } else {
+      cj = K[i];
+      ev = V[i];
+    }
+    if (cj != MGL_EMPTY) {
+      ++cnt;
+      const double wij = gain((double)ev, deici, dki, daci, (double)a[cj], Mh);
+      if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+    }
+  }
+  block_argmax<BS>(best, dst, small, rg, rd);
+  if (A.stats) cnt = block_sum<BS>(cnt, rc);
+  if (threadIdx.x == 0) {
+    A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
+    if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
+    if (A.stats) atomicAdd(A.stats, (unsigned long long)cnt);
+  }
+}
+
+// ---------------------------------------------------------------------------
+BinLayout default_bins() {
+  BinLayout L;
+  int i = 0;
+  L.b[i++] = {0, SHAPE_NONE, 0, 0};
+  L.b[i++] = {4, SHAPE_GROUP, 4, 3};
+  L.b[i++] = {8, SHAPE_GROUP, 8, 4};
+  L.b[i++] = {16, SHAPE_GROUP, 16, 5};
+  L.b[i++] = {32, SHAPE_GROUP, 32, 6};
+  L.b[i++] = {64, SHAPE_GROUP, 64, 7};
+  L.b[i++] = {128, SHAPE_GROUP, 64, 8};
+  L.b[i++] = {256, SHAPE_GROUP, 64, 9};
+  L.b[i++] = {1024, SHAPE_BLOCK, 256, 11};
+  L.b[i++] = {4096, SHAPE_BLOCK, 1024, 13};
+  L.b[i++] = {INT64_MAX, SHAPE_GLOBAL, 1024, 0};
+  L.nb = i;
+  return L;
+}
+
+template <class M>
+static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int64_t nb,
+                      const int64_t* toff, vid* gk, void* gv, hipStream_t st) {
+  typedef typename M::S ST;
+  if (nb <= 0 || bs.shape == SHAPE_NONE) return 0;
+  if (bs.shape == SHAPE_GROUP) {
+    const int gpb = 256 / bs.s;
+    const unsigned grid = (unsigned)((nb + gpb - 1) / gpb);
+#define G_(S_, LG_)                                                                     \
+  if (bs.s == S_ && bs.lg == LG_) {                                                     \
+    k_move_group<M, S_, LG_><<<grid, 256, 0, st>>>(A, perm, nb);                        \
+    return 0;                                                                           \
+  }
+    G_(4, 3) G_(4, 4) G_(8, 4) G_(8, 5) G_(16, 5) G_(16, 6) G_(32, 6) G_(32, 7)
+    G_(64, 7) G_(64, 8) G_(64, 9) G_(64, 10)
+#undef G_
+  } else if (bs.shape == SHAPE_BLOCK) {
+    const size_t lds = (((size_t)4 << bs.lg) + 15) / 16 * 16 + (sizeof(ST) << bs.lg);
+    if (bs.s == 256) {
+      k_move_block<M, 256, false><<<(unsigned)nb, 256, lds, st>>>(A, perm, nb, bs.lg, nullptr,
+                                                                   nullptr, nullptr);
+      return 0;
+    }
+    if (bs.s == 512) {
+      k_move_block<M, 512, false><<<(unsigned)nb, 512, lds, st>>>(A, perm, nb, bs.lg, nullptr,
+                                                                   nullptr, nullptr);
+      return 0;
+    }
+    if (bs.s == 1024) {
+      k_move_block<M, 1024, false><<<(unsigned)nb, 1024, lds, st>>>(A, perm, nb, bs.lg,
+                                                                     nullptr, nullptr, nullptr);
+      return 0;
+    }
+  } else if (bs.shape == SHAPE_GLOBAL) {
+    k_move_block<M, 1024, true><<<(unsigned)nb, 1024, 0, st>>>(A, perm, nb, 0, toff, gk,
+                                                                (ST*)gv);
+    return 0;
+  }
+  return -1;
+}
+
+int launch_move_bin(int mode, const BinSpec& bs, const MoveArgs& A, const vid* perm,
+                    int64_t nb, const int64_t* toff, vid* gk, void* gv, hipStream_t st) {
+  switch (mode) {
+    case MODE_UNIT: return move_bin_t<ModeUnit>(bs, A, perm, nb, toff, gk, gv, st);
+    case MODE_INT: return move_bin_t<ModeInt>(bs, A, perm, nb, toff, gk, gv, st);
+    case MODE_F64: return move_bin_t<ModeF64>(bs, A, perm, nb, toff, gk, gv, st);
+    case MODE_F32: return move_bin_t<ModeF32>(bs, A, perm, nb, toff, gk, gv, st);
+  }
+  return -1;
+}
+
+// Opt the LDS-table kernels into the device's largest workgroup LDS size.
+void move_kernels_allow_lds(int bytes) {
+#define A_(M_, BS_) \
+  (void)hipFuncSetAttribute((const void*)k_move_block<M_, BS_, false>, \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  A_(ModeUnit, 256) A_(ModeUnit, 512) A_(ModeUnit, 1024)
+  A_(ModeInt, 256) A_(ModeInt, 512) A_(ModeInt, 1024)
+  A_(ModeF64, 256) A_(ModeF64, 512) A_(ModeF64, 1024)
+  A_(ModeF32, 256) A_(ModeF32, 512) A_(ModeF32, 1024)
+#undef A_
+}
+
+// ---------------------------------------------------------------------------
+// Row reductions: S lanes per vertex (S <= 64) or a workgroup (S = 256).
+//   OP_WEIGHT: k[v] = sum_row w (self-loops included, louvain.cu:30-74),
+//              vin[v] = self-loop weight
+//   OP_INTRA : vin[v] = sum of w over entries with c[u] == c[v] (self-loop
+//              included, louvain.cu:187-206)
+template <class M, int S, int OP>
+__global__ __launch_bounds__(256) void k_rows(RowArgs A, const vid* __restrict__ perm, int64_t nb) {
+  typedef typename M::K KT;
+  constexpr int GPB = (S >= 256) ? 1 : 256 / S;
+  __shared__ KT red[4];
+  const int grp = (S >= 256) ? 0 : threadIdx.x / S;
+  const int lane = (S >= 256) ? threadIdx.x : (threadIdx.x & (S - 1));
+  const int64_t gi = (int64_t)blockIdx.x * GPB + grp;
+  bool ok = gi < nb;
+  vid v = 0;
+  if (ok) {
+    v = perm[gi];
+    if (OP == OP_INTRA && A.gs) {
+      const int g = A.gid ? A.gid[v] : 0;
+      ok = A.gs[g].active != 0;
+    }
+  }
+  if (S >= 256 && !ok) return;  // uniform per workgroup
+  KT s0 = 0, s1 = 0;
+  if (ok) {
+    const eid e0 = A.off[v], e1 = A.off[v + 1];
+    const vid cv = (OP == OP_INTRA) ? A.c[v] : 0;
+    for (eid e = e0 + lane; e < e1; e += (S >= 256 ? 256 : S)) {
+      const vid u = A.col[e];
+      const KT wv = (KT)wload<M>(A.w, e);
+      if (OP == OP_WEIGHT) {
+        s0 += wv;
+        if (u == v) s1 += wv;
+      } else {
+        if (A.c[u] == cv) s1 += wv;
+      }
+    }
+  }
+  if (S >= 256) {
+    s0 = block_sum<256>(s0, red);
+    s1 = block_sum<256>(s1, red);
+  } else {
+    if (!ok) return;
+    s0 = group_sum<S>(s0);
+    s1 = group_sum<S>(s1);
+  }
+  if (lane == 0) {
+    if (OP == OP_WEIGHT) ((KT*)A.k)[v] = s0;
+    ((KT*)A.vin)[v] = s1;
+  }
+}
+
+template <class M, int OP>
+static void rows_bin_t(const BinSpec& bs, const RowArgs& A, const vid* perm, int64_t nb,
+                       hipStream_t st) {
+  if (nb <= 0 || bs.shape == SHAPE_NONE) return;
+  if (bs.shape == SHAPE_GROUP && bs.upper <= 4) {
+    k_rows<M, 4, OP><<<(unsigned)((nb + 63) / 64), 256, 0, st>>>(A, perm, nb);
+  } else if (bs.shape == SHAPE_GROUP && bs.upper <= 16) {
+    k_rows<M, 16, OP><<<(unsigned)((nb + 15) / 16), 256, 0, st>>>(A, perm, nb);
+  } else if (bs.shape == SHAPE_GROUP) {
+    k_rows<M, 64, OP><<<(unsigned)((nb + 3) / 4), 256, 0, st>>>(A, perm, nb);
+  } else {
+    k_rows<M, 256, OP><<<(unsigned)nb, 256, 0, st>>>(A, perm, nb);
+  }
+}
+
+void launch_rows_bin(int mode, int op, const BinSpec& bs, const RowArgs& A, const vid* perm,
+                     int64_t nb, hipStream_t st) {
+#define R_(M_)                                                  \
+  if (op == OP_WEIGHT) rows_bin_t<M_, OP_WEIGHT>(bs, A, perm, nb, st); \
+  else rows_bin_t<M_, OP_INTRA>(bs, A, perm, nb, st);
+  switch (mode) {
+    case MODE_UNIT: R_(ModeUnit) break;
+    case MODE_INT: R_(ModeInt) break;
+    case MODE_F64: R_(ModeF64) break;
+    case MODE_F32: R_(ModeF32) break;
+  }
+#undef R_
+}
+
+// ---------------------------------------------------------------------------
+// Degree binning (BIN.cu:33-121 histogram, :298-342 permutation). Within a
+// bin the order is whatever the atomics give; no result depends on it.
+struct BinBounds {
+  int64_t up[MGL_MAX_BINS];
+  int nb;
+};
+__device__ __forceinline__ int bin_of(int64_t d, const BinBounds& B) {
+  int b = 0;
+  while (b < B.nb - 1 && d > B.up[b]) ++b;
+  return b;
+}
+static BinBounds bounds_of(const BinLayout& L) {
+  BinBounds B;
+  B.nb = L.nb;
+  for (int i = 0; i < L.nb; ++i) B.up[i] = L.b[i].upper;
+  return B;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const eid* __restrict__ off, int64_t n,
+                                                   BinBounds B,
+                                                   unsigned long long* __restrict__ counts) {
+  __shared__ unsigned int h[MGL_MAX_BINS];
+  if (threadIdx.x < MGL_MAX_BINS) h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256)
+    atomicAdd(&h[bin_of(off[v + 1] - off[v], B)], 1u);
+  __syncthreads();
+  if (threadIdx.x < B.nb && h[threadIdx.x])
+    atomicAdd(&counts[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(const eid* __restrict__ off, int64_t n,
+                                                     BinBounds B,
+                                                     unsigned long long* __restrict__ cursor,
+                                                     vid* __restrict__ perm) {
+  __shared__ unsigned int h[MGL_MAX_BINS];
+  __shared__ unsigned long long base[MGL_MAX_BINS];
+  for (int64_t v0 = (int64_t)blockIdx.x * 256; v0 < n; v0 += (int64_t)gridDim.x * 256) {
+    if (threadIdx.x < MGL_MAX_BINS) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t v = v0 + threadIdx.x;
+    int b = -1;
+    unsigned int r = 0;
+    if (v < n) {
+      b = bin_of(off[v + 1] - off[v], B);
+      r = atomicAdd(&h[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < B.nb && h[threadIdx.x])
+      base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+    __syncthreads();
+    if (b >= 0) perm[base[b] + r] = (vid)v;
+    __syncthreads();
+  }
+}
+
+static unsigned grid_for(int64_t n, int per_block, unsigned cap) {
+  int64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+void launch_bin_count(const eid* off, int64_t n, const BinLayout& L, unsigned long long* counts,
+                      hipStream_t st) {
+  if (n <= 0) return;
+  k_bin_count<<<grid_for(n, 256, 4096), 256, 0, st>>>(off, n, bounds_of(L), counts);
+}
+void launch_bin_scatter(const eid* off, int64_t n, const BinLayout& L, unsigned long long* cursor,
+                        vid* perm, hipStream_t st) {
+  if (n <= 0) return;
+  k_bin_scatter<<<grid_for(n, 256, 4096), 256, 0, st>>>(off, n, bounds_of(L), cursor, perm);
+}
+
+// hash-table slots of a global-hash vertex: 2 * nextpow2(degree), at least 64
+__global__ void k_table_sizes(const eid* off, const vid* perm, int64_t nb, int64_t* sizes) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nb) return;
+  const vid v = perm[i];
+  int64_t d = off[v + 1] - off[v];
+  int64_t p = 64;
+  while (p < 2 * d) p <<= 1;
+  sizes[i] = p;
+}
+void launch_table_sizes(const eid* off, const vid* perm, int64_t nb, int64_t* sizes,
+                        hipStream_t st) {
+  if (nb <= 0) return;
+  k_table_sizes<<<grid_for(nb, 256, 1u << 30), 256, 0, st>>>(off, perm, nb, sizes);
+}
+
+// ---------------------------------------------------------------------------
+// Community-weight update: a[c] -= k, a[c_new] += k for every moved vertex of
+// an active graph (louvain_baseline.cu:1339-1361 form; louvain.cu:1328-1336
+// with 1377-1381 routes the same integers through a zeroed delta buffer).
+template <class KT>
+__global__ __launch_bounds__(256) void k_update(int64_t n, const vid* __restrict__ c,
+                                                const vid* __restrict__ cn,
+                                                const KT* __restrict__ k, KT* a,
+                                                const int32_t* __restrict__ gid, GState* gs) {
+  __shared__ unsigned int moved;
+  if (threadIdx.x == 0) moved = 0;
+  __syncthreads();
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
+    const vid x = c[v], y = cn[v];
+    if (x != y) {
+      const int g = gid ? gid[v] : 0;
+      if (gs[g].active) {
+        const KT kv = k[v];
+        atom_add(&a[x], -kv);
+        atom_add(&a[y], kv);
+        atomicAdd(&moved, 1u);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && moved) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)moved);
+}
+
+void launch_update(int mode, int64_t n, const vid* c, const vid* cn, const void* k, void* a,
+                   const int32_t* gid, GState* gs, hipStream_t st) {
+  if (n <= 0) return;
+  const unsigned grid = grid_for(n, 256, 8192);
+  if (mode == MODE_UNIT || mode == MODE_INT)
+    k_update<int64_t><<<grid, 256, 0, st>>>(n, c, cn, (const int64_t*)k, (int64_t*)a, gid, gs);
+  else
+    k_update<double><<<grid, 256, 0, st>>>(n, c, cn, (const double*)k, (double*)a, gid, gs);
+}
+
+// ---------------------------------------------------------------------------
+// Per-graph reduction of sum_in (from vin) and sum_c a_c^2 (louvain.cu:146-165,
+// 187-206). Graphs occupy contiguous vertex ranges, so each wavefront walks a
+// contiguous range, keeps lane-private partials for the current graph and
+// flushes them (exact 64/128-bit atomics, or FP64 atomics) when the graph id
+// changes and at the end.
+template <bool EXACT>
+struct RedAcc;
+template <>
+struct RedAcc<true> {
+  int64_t sin = 0;
+  u128 sa2 = 0;
+  __device__ void add(int64_t vin, int64_t a) {
+    sin += vin;
+    const u128 x = (u128)(uint64_t)(a < 0 ? -a : a);
+    sa2 += x * x;
+  }
+  __device__ void flush(GState* s) {
+    int64_t t = sin;
+    uint64_t lo = (uint64_t)sa2, hi = (uint64_t)(sa2 >> 64);
+    t = group_sum<64>(t);
+    // 128-bit wave sum: sum the halves separately with carries
+    uint64_t lo_sum = lo, hi_sum = hi;
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      uint64_t olo = (uint64_t)shfl_xor_t((int64_t)lo_sum, m, 64);
+      uint64_t ohi = (uint64_t)shfl_xor_t((int64_t)hi_sum, m, 64);
+      uint64_t nlo = lo_sum + olo;
+      hi_sum = hi_sum + ohi + (nlo < lo_sum ? 1ull : 0ull);
+      lo_sum = nlo;
+    }
+    if ((threadIdx.x & 63) == 0) {
+      if (t) atomicAdd((unsigned long long*)&s->sin_i, (unsigned long long)t);
+      atomic_add_u128(&s->sa2_lo, &s->sa2_hi, ((u128)hi_sum << 64) | lo_sum);
+    }
+    sin = 0;
+    sa2 = 0;
+  }
+};
+template <>
+struct RedAcc<false> {
+  double sin = 0, sa2 = 0;
+  __device__ void add(double vin, double a) { sin += vin; sa2 += a * a; }
+  __device__ void flush(GState* s) {
+    double t = group_sum<64>(sin), u = group_sum<64>(sa2);
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&s->sin_d, t);
+      atomicAdd(&s->sa2_d, u);
+    }
+    sin = 0;
+    sa2 = 0;
+  }
+};
+
+template <class KT, bool EXACT>
+__global__ __launch_bounds__(256) void k_reduce(int64_t n, const KT* __restrict__ vin,
+                                                const KT* __restrict__ a,
+                                                const int32_t* __restrict__ gid, GState* gs,
+                                                int64_t per_wave) {
+  const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t beg = wave * per_wave;
+  if (beg >= n) return;
+  const int64_t end = beg + per_wave < n ? beg + per_wave : n;
+  RedAcc<EXACT> acc;
+  int gcur = gid ? gid[beg] : 0;
+  for (int64_t v0 = beg; v0 < end; v0 += 64) {
+    const int64_t v = v0 + lane;
+    const bool ok = v < end;
+    int g = gcur;
+    if (gid) {
+      g = ok ? gid[v] : gid[end - 1];
+      const int gf = __shfl(g, 0);
+      const int gl = __shfl(g, 63);
+      if (gf != gcur || gl != gcur) {
+        // the wave straddles a graph boundary: flush, then handle lane by lane
+        acc.flush(&gs[gcur]);
+        if (gf == gl) {
+          gcur = gf;
+        } else {
+          for (int j = 0; j < 64; ++j) {
+            const int gj = __shfl(g, j);
+            RedAcc<EXACT> one;
+            if (lane == j && ok && gs[gj].active) one.add(vin ? vin[v] : (KT)0, a[v]);
+            one.flush(&gs[gj]);
+          }
+          gcur = gl;
+          continue;
+        }
+      }
+    }
+    if (ok && gs[gcur].active) acc.add(vin ? vin[v] : (KT)0, a[v]);
+  }
+  acc.flush(&gs[gcur]);
+}
+
+void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const int32_t* gid,
+                   GState* gs, hipStream_t st) {
+  if (n <= 0) return;
+  int64_t waves = (n + 1023) / 1024;  // >= 1024 vertices per wave
+  if (waves > 4096) waves = 4096;
+  int64_t per_wave = (n + waves - 1) / waves;
+  per_wave = (per_wave + 63) / 64 * 64;
+  waves = (n + per_wave - 1) / per_wave;
+  const unsigned grid = (unsigned)((waves + 3) / 4);
+  if (mode == MODE_UNIT || mode == MODE_INT)
+    k_reduce<int64_t, true><<<grid, 256, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a,
+                                                  gid, gs, per_wave);
+  else
+    k_reduce<double, false><<<grid, 256, 0, st>>>(n, (const double*)vin, (const double*)a, gid,
+                                                  gs, per_wave);
+}
+
+// ---------------------------------------------------------------------------
+// Inner-loop control, one thread per graph (louvain.cu:1548-1660).
+__device__ __forceinline__ double q_of(const GState& s, bool exact, bool prev) {
+  const double M = s.mass;
+  if (M == 0.0) return 0.0;
+  double sin, sa2;
+  if (exact) {
+    sin = (double)s.sin_i;
+    sa2 = prev ? u128_to_double(s.sa2p_hi, s.sa2p_lo) : u128_to_double(s.sa2_hi, s.sa2_lo);
+  } else {
+    sin = s.sin_d;
+    sa2 = prev ? s.sa2p_d : s.sa2_d;
+  }
+  return sin / M - sa2 / (M * M);  // louvain.cu:234
+}
+
+__device__ __forceinline__ void reset_acc(GState& s) {
+  s.sin_i = 0; s.sin_d = 0; s.sa2_lo = 0; s.sa2_hi = 0; s.sa2_d = 0;
+}
+
+// Level start, step 1: graphs listed in run[] become active, accumulators cleared.
+__global__ void k_level_begin(int ng, GState* gs, const int32_t* run) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  GState& s = gs[g];
+  s.active = run[g] ? 1 : 0;
+  s.iters = 0;
+  s.fix = 0;
+  reset_acc(s);
+  s.sa2p_lo = 0; s.sa2p_hi = 0; s.sa2p_d = 0;
+}
+void launch_level_begin(int ng, GState* gs, const int32_t* run, hipStream_t st) {
+  k_level_begin<<<(ng + 255) / 256, 256, 0, st>>>(ng, gs, run);
+}
+
+// Level start, step 2: q_cur = Q(singletons) from the accumulators
+// (louvain.cu:1509-1533).
+__global__ void k_level_init(int ng, GState* gs, int exact) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  GState& s = gs[g];
+  if (s.active) s.q_cur = q_of(s, exact != 0, false);
+  reset_acc(s);
+}
+void launch_level_init(int ng, GState* gs, int exact, hipStream_t st) {
+  k_level_init<<<(ng + 255) / 256, 256, 0, st>>>(ng, gs, exact);
+}
+
+// pre = 1: after the first move of a level (fused mode) only stores sum a^2 of
+// the first proposal; no decision is taken.
+__global__ void k_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
+                         double* traj, int traj_max, int pre) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  GState& s = gs[g];
+  if (!s.active) { s.fix = 0; reset_acc(s); return; }
+  if (pre) {
+    s.sa2p_lo = s.sa2_lo; s.sa2p_hi = s.sa2_hi; s.sa2p_d = s.sa2_d;
+    s.fix = 0;
+    reset_acc(s);
+    return;
+  }
+  const double Q = q_of(s, exact != 0, fused != 0);
+  const int it = s.iters++;
+  if ((Q - s.q_cur) > thr) {
+    s.q_cur = Q;
+    if (traj && it < traj_max) traj[(int64_t)g * traj_max + it] = Q;
+    const bool stop = (max_iter > 0 && s.iters >= max_iter);
+    if (fused) s.fix = stop ? 2 : 0;
+    else s.fix = 3;
+    if (stop) s.active = 0;
+  } else {
+    if (traj && it < traj_max) traj[(int64_t)g * traj_max + it] = s.q_cur;
+    s.active = 0;
+    s.fix = fused ? 1 : 0;
+  }
+  if (fused) { s.sa2p_lo = s.sa2_lo; s.sa2p_hi = s.sa2_hi; s.sa2p_d = s.sa2_d; }
+  reset_acc(s);
+}
+void launch_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
+                   double* traj, int traj_max, int pre, hipStream_t st) {
+  k_decide<<<(ng + 255) / 256, 256, 0, st>>>(ng, gs, thr, max_iter, fused, exact, traj, traj_max,
+                                             pre);
+}
+
+// Buffer fix-up after a decision:
+//   fix 1 (fused, rejected)        : c, p, n := c
+//   fix 2 (fused, accepted, stop)  : c, p, n := p
+//   fix 3 (unfused, accepted)      : c := p
+__global__ __launch_bounds__(256) void k_fixup(int64_t n, vid* c, vid* p, vid* nx,
+                                               const int32_t* __restrict__ gid,
+                                               const GState* __restrict__ gs) {
+  if (!gid && gs[0].fix == 0) return;
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
+    const int g = gid ? gid[v] : 0;
+    const int f = gs[g].fix;
+    if (f == 0) continue;
+    if (f == 3) { c[v] = p[v]; continue; }
+    const vid x = (f == 1) ? c[v] : p[v];
+    c[v] = x;
+    p[v] = x;
+    if (nx) nx[v] = x;
+  }
+}
+void launch_fixup(int64_t n, vid* c, vid* p, vid* nx, const int32_t* gid, const GState* gs,
+                  hipStream_t st) {
+  if (n <= 0) return;
+  k_fixup<<<grid_for(n, 256, 8192), 256, 0, st>>>(n, c, p, nx, gid, gs);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_iota(vid* x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = (vid)i;
+}
+void launch_iota(vid* x, int64_t n, hipStream_t st) {
+  if (n > 0) k_iota<<<grid_for(n, 256, 8192), 256, 0, st>>>(x, n);
+}
+template <class T>
+__global__ void k_copy(const T* s, T* d, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+void launch_copy_k_to_a(int mode, int64_t n, const void* k, void* a, hipStream_t st) {
+  if (n <= 0) return;
+  if (mode == MODE_UNIT || mode == MODE_INT)
+    k_copy<int64_t><<<grid_for(n, 256, 8192), 256, 0, st>>>((const int64_t*)k, (int64_t*)a, n);
+  else
+    k_copy<double><<<grid_for(n, 256, 8192), 256, 0, st>>>((const double*)k, (double*)a, n);
+}
+void launch_copy_vid(const vid* s, vid* d, int64_t n, hipStream_t st) {
+  if (n > 0) k_copy<vid><<<grid_for(n, 256, 8192), 256, 0, st>>>(s, d, n);
+}
+__global__ void k_zero(uint32_t* p, int64_t nw) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nw;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+void launch_fill_bytes0(void* p, size_t bytes, hipStream_t st) {
+  // a fill kernel (not hipMemsetAsync) so the sequence can be graph-captured
+  const int64_t nw = (int64_t)((bytes + 3) / 4);
+  if (nw > 0) k_zero<<<grid_for(nw, 256, 8192), 256, 0, st>>>((uint32_t*)p, nw);
+}
+
+// float input weights -> level-0 storage of the chosen mode. MODE_INT scales
+// by 2^scale (exact for dyadic weights, checked on the host).
+__global__ void k_cast(int to_mode, const float* w, void* out, int64_t nnz, double mul) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double x = (double)w[i];
+    if (to_mode == MODE_INT) ((int64_t*)out)[i] = (int64_t)(x * mul);
+    else if (to_mode == MODE_F64) ((double*)out)[i] = x;
+    else ((float*)out)[i] = w[i];
+  }
+}
+void launch_cast_weights(int to_mode, const float* w, void* out, int64_t nnz, int scale,
+                         hipStream_t st) {
+  if (nnz <= 0) return;
+  double mul = 1.0;
+  for (int i = 0; i < scale; ++i) mul *= 2.0;
+  k_cast<<<grid_for(nnz, 256, 8192), 256, 0, st>>>(to_mode, w, out, nnz, mul);
+}
+
+__global__ void k_gid_fill(const int64_t* voff, int ng, int32_t* gid) {
+  const int g = blockIdx.x;
+  if (g >= ng) return;
+  for (int64_t v = voff[g] + threadIdx.x; v < voff[g + 1]; v += blockDim.x) gid[v] = g;
+}
+void launch_gid_fill(const int64_t* voff, int ng, int32_t* gid, hipStream_t st) {
+  if (ng > 0) k_gid_fill<<<ng, 256, 0, st>>>(voff, ng, gid);
+}

@@ -1,0 +1,172 @@
+"""GPU parity: the HIP path (through libmgl.so's C ABI) against the CPU oracle.
+
+Exact modes must be bit-exact: community assignments, coarse CSRs, iteration
+counts and every modularity value of the trajectory. Inputs are seeded
+synthetic graphs at sizes the oracle finishes in seconds, plus hand-built edge
+cases (empty graph, isolated vertices, self-loops, stars whose hub takes the
+HBM-hash bin).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def two_triangles(mgl):
+    return mgl.CSR.from_edges(6, [(0, 1), (0, 2), (1, 2), (3, 4), (3, 5), (4, 5), (2, 3)])
+
+
+def compare(gpu, ref, check_graphs=True, scale=0):
+    assert len(gpu.levels) == len(ref.levels), (len(gpu.levels), len(ref.levels))
+    for l, (g, r) in enumerate(zip(gpu.levels, ref.levels)):
+        assert g.n == r["n"], (l, g.n, r["n"])
+        assert g.iters == r["iters"], (l, g.iters, r["iters"])
+        assert g.q == r["q"], (l, g.q, r["q"])
+        np.testing.assert_array_equal(g.qtraj, r["qtraj"])
+        if g.assign is not None:
+            np.testing.assert_array_equal(g.assign, r["assign"])
+        if check_graphs and l >= 1 and g.graph is not None:
+            go, gc, gw = ref.graphs[l]
+            np.testing.assert_array_equal(g.graph.offsets, go)
+            np.testing.assert_array_equal(g.graph.cols, gc)
+            np.testing.assert_array_equal(g.graph_w64, np.ldexp(gw, -scale))
+    np.testing.assert_array_equal(gpu.final_assign, ref.final)
+
+
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_two_triangles_known_answer(ctx, mgl, version):
+    # hand-derived known answer (two triangles joined by edge 2-3)
+    r = ctx.louvain(two_triangles(mgl), version=version, keep_levels=True, keep_graphs=True)
+    assert len(r.levels) == 2
+    l0 = r.levels[0]
+    # exact rationals; the FP64 expression Q = sum_in/2m - sum a^2/(2m)^2 may
+    # differ from the correctly rounded fraction in the last place
+    if version == 1:
+        assert l0.qtraj[0] == pytest.approx(18 / 196, abs=1e-15)
+    else:
+        assert l0.qtraj[0] == pytest.approx(38 / 196, abs=1e-15)
+    assert l0.qtraj[1] == pytest.approx(5 / 14, abs=1e-15)
+    assert l0.q == pytest.approx(5 / 14, abs=1e-15)
+    np.testing.assert_array_equal(l0.assign, [0, 0, 0, 4, 4, 4])
+    g1 = r.levels[1].graph
+    np.testing.assert_array_equal(g1.offsets, [0, 2, 4])
+    np.testing.assert_array_equal(g1.cols, [0, 1, 0, 1])
+    np.testing.assert_array_equal(r.levels[1].graph_w64, [6, 1, 1, 6])
+    assert r.levels[1].q == pytest.approx(5 / 14, abs=1e-15)
+    np.testing.assert_array_equal(r.final_assign, [0, 0, 0, 1, 1, 1])
+
+
+@pytest.mark.parametrize("version", [0, 1, 2])
+@pytest.mark.parametrize("fused", [True, False])
+def test_rmat16_full_parity(ctx, mgl, orc, version, fused):
+    g = mgl.gen_rmat(16, 16, seed=1)
+    print(f"rmat16 v{version} fused={fused} n={g.n} nnz={g.nnz}", flush=True)
+    res = ctx.louvain(g, version=version, fuse_modularity=fused, keep_levels=True,
+                      keep_graphs=True)
+    ref = orc.louvain(g.offsets, g.cols, None, mode=0, version=version)
+    compare(res, ref)
+
+
+def test_move_stage_random_assignments(ctx, mgl, orc):
+    g = mgl.gen_rmat(14, 16, seed=7)
+    s = ctx.session(g)
+    rng = np.random.default_rng(0)
+    k = orc.vertex_weights(g.offsets)
+    for trial in range(4):
+        # random coarse-grained assignment: community = id of a random member
+        c = rng.integers(0, g.n, g.n).astype(np.int32)
+        c = c[c]  # ids that are themselves used ids more often
+        s.set_assign(c)
+        a = orc.community_totals(k, c, g.n)
+        ga, gk = s.weights()
+        np.testing.assert_array_equal(gk, k)
+        np.testing.assert_array_equal(ga, a)
+        for up in (True, False):
+            got = s.move(up)
+            want = orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up)
+            np.testing.assert_array_equal(got, want)
+        assert s.modularity() == orc.modularity(g.offsets, g.cols, None, c, a, g.nnz)
+    s.close()
+
+
+def test_coarsen_stage(ctx, mgl, orc):
+    g = mgl.gen_rmat(14, 16, seed=9)
+    s = ctx.session(g)
+    rng = np.random.default_rng(3)
+    c = rng.integers(0, g.n // 8, g.n).astype(np.int32) * 8
+    s.set_assign(c)
+    off2, col2, w2, rank = s.coarsen()
+    ro, rc, rw, rr = orc.coarsen(g.offsets, g.cols, None, c)
+    np.testing.assert_array_equal(off2, ro)
+    np.testing.assert_array_equal(col2, rc)
+    np.testing.assert_array_equal(w2, rw.astype(np.float64))
+    np.testing.assert_array_equal(rank, rr)
+    s.close()
+
+
+def test_hub_uses_global_bin(ctx, mgl, orc):
+    # a star with 9000 leaves plus a ring: the centre's degree exceeds the
+    # LDS-table bins, so its row goes through the HBM hash
+    n = 9001
+    edges = [(0, i) for i in range(1, n)] + [(i, i + 1) for i in range(1, n - 1)]
+    g = mgl.CSR.from_edges(n, edges)
+    res = ctx.louvain(g, keep_levels=True, keep_graphs=True)
+    ref = orc.louvain(g.offsets, g.cols, None)
+    compare(res, ref)
+    s = ctx.session(g)
+    bins = s.bins()
+    assert bins[-1] == 1
+    s.close()
+
+
+def test_edge_cases(ctx, mgl, orc):
+    cases = {
+        "isolated_only": mgl.CSR(np.zeros(6, np.int64), np.zeros(0, np.int32)),
+        "single_edge": mgl.CSR.from_edges(2, [(0, 1)]),
+        "self_loops": mgl.CSR.from_edges(5, [(0, 0), (0, 1), (1, 2), (2, 2), (3, 4), (4, 4)]),
+        "with_isolated": mgl.CSR.from_edges(7, [(0, 1), (1, 2), (2, 0), (4, 5)]),
+    }
+    for name, g in cases.items():
+        print("edge case", name, flush=True)
+        res = ctx.louvain(g, keep_levels=True, keep_graphs=True)
+        ref = orc.louvain(g.offsets, g.cols, None)
+        compare(res, ref)
+
+
+def test_empty_graph(ctx, mgl):
+    g = mgl.CSR(np.zeros(1, np.int64), np.zeros(0, np.int32))
+    res = ctx.louvain(g)
+    assert len(res.levels) == 1 and res.levels[0].n == 0 and res.final_q == 0.0
+
+
+def test_sbm_weighted_exact(ctx, mgl, orc):
+    g = mgl.gen_sbm(1 << 14, 64, min_size=64, seed=5, weights=1)
+    S = orc.scale_of(g.w)
+    assert S == 24
+    res = ctx.louvain(g, precision=mgl.PREC_EXACT, keep_levels=True, keep_graphs=True)
+    assert res.precision == mgl.PREC_EXACT
+    ref = orc.louvain(g.offsets, g.cols, g.w.astype(np.float64), mode=0)
+    compare(res, ref, scale=S)
+
+
+def test_batched_equals_plain(ctx, mgl):
+    graphs = [mgl.gen_rmat(10 + (i % 3), 8 + 4 * (i % 3), seed=100 + i) for i in range(5)]
+    graphs.append(two_triangles(mgl))
+    graphs.append(mgl.gen_sbm(3000, 12, min_size=64, seed=11, weights=0, deg_in=16, deg_out=3))
+    batch = ctx.louvain_batched(graphs, keep_levels=True)
+    for g, b in zip(graphs, batch):
+        p = ctx.louvain(g, keep_levels=True)
+        assert len(b.levels) == len(p.levels)
+        for lb, lp in zip(b.levels, p.levels):
+            assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
+            np.testing.assert_array_equal(lb.qtraj, lp.qtraj)
+            np.testing.assert_array_equal(lb.assign, lp.assign)
+        np.testing.assert_array_equal(b.final_assign, p.final_assign)
+
+
+def test_repeat_runs_identical(ctx, mgl):
+    g = mgl.gen_rmat(15, 16, seed=21)
+    a = ctx.louvain(g, keep_levels=True)
+    b = ctx.louvain(g, keep_levels=True)
+    assert a.final_q == b.final_q
+    np.testing.assert_array_equal(a.final_assign, b.final_assign)
