@@ -132,6 +132,7 @@ typedef struct {
   int64_t dominant_vertices, dominant_entries; /* vertices / CSR entries that launch processed */
   int64_t moved;       /* vertices whose proposal differs from their community */
   int64_t distinct;    /* (vertex, neighbour community) pairs aggregated in the move */
+  int64_t dominant_distinct; /* the same, for the dominant launch */
   double q;            /* committed modularity */
 } mgl_pass_stats;
 
@@ -197,6 +198,8 @@ void mgl_csr_free(mgl_host_csr* g);
 int mgl_version_major(void);
 /* Number of move bins and their upper degree bounds (inclusive). */
 int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins);
+/* LDS bytes one workgroup may use on the context's device (sizes the tables). */
+int mgl_device_lds_bytes(const mgl_ctx* ctx);
 /* Times the move pass of g under candidate bin layouts and writes the fastest
  * to path (JSON lines). */
 mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* path);

@@ -69,7 +69,8 @@ class PassStats(C.Structure):
                 ("pass_ms", C.c_double), ("dominant_ms", C.c_double),
                 ("dominant_bin", C.c_int), ("dominant_vertices", C.c_int64),
                 ("dominant_entries", C.c_int64), ("moved", C.c_int64),
-                ("distinct", C.c_int64), ("q", C.c_double)]
+                ("distinct", C.c_int64), ("dominant_distinct", C.c_int64),
+                ("q", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -124,6 +125,7 @@ _sig("mgl_csr_free", None, [C.POINTER(_HostCsr)])
 _sig("mgl_version_major", _i, [])
 _sig("mgl_bin_bounds", _i, [_vp, C.POINTER(_i64), _i])
 _sig("mgl_autotune", _i, [_vp, C.POINTER(_Csr), C.c_char_p])
+_sig("mgl_device_lds_bytes", _i, [_vp])
 
 
 def _p(a, t):
@@ -328,6 +330,9 @@ class Context:
         buf = (C.c_int64 * 16)()
         nb = _lib.mgl_bin_bounds(self._h, buf, 16)
         return [buf[i] for i in range(nb)]
+
+    def lds_bytes(self) -> int:
+        return _lib.mgl_device_lds_bytes(self._h)
 
     def autotune(self, g: CSR, path: str):
         cg = g._c()

@@ -90,6 +90,15 @@ __device__ __forceinline__ uint32_t slot_hash(vid key, int lg) {
   return ((uint32_t)key * 2654435761u) >> (32 - lg);
 }
 
+// Slot of a key in a table of H slots (any H): a mixed 32-bit hash scaled by
+// a multiply-high, so no power-of-two size is needed.
+__device__ __forceinline__ uint32_t any_slot(vid key, uint32_t H) {
+  uint32_t x = (uint32_t)key * 0x85EBCA6Bu;
+  x ^= x >> 15;
+  x *= 0xC2B2AE35u;
+  return (uint32_t)(((uint64_t)x * H) >> 32);
+}
+
 // Total order used by the argmax (louvain.cu:544-554): higher gain first; on
 // equal gain the smaller id on up passes and the larger id on down passes
 // (tie_small forces "smaller" always).

@@ -36,6 +36,7 @@ struct mgl_ctx {
   int lds_max = 65536;
   BinLayout user_bins;
   bool have_user_bins = false;
+  int64_t hub_bucket = 4096;  // hub candidates per bucket on average
 };
 
 #define FAIL(ctx, code, ...)                                    \
@@ -104,20 +105,22 @@ static int slot_bytes(int mode) {
 
 // Bin layout for a mode, with LDS tables shrunk to what the device allows.
 static BinLayout layout_for(const mgl_ctx* ctx, int mode) {
-  BinLayout L = ctx->have_user_bins ? ctx->user_bins : default_bins();
+  const int sb = slot_bytes(mode);
+  BinLayout L = ctx->have_user_bins ? ctx->user_bins : default_bins(ctx->lds_max, sb);
   for (int i = 0; i < L.nb; ++i) {
     BinSpec& b = L.b[i];
     if (b.shape == SHAPE_BLOCK) {
-      while (b.lg > 4 &&
-             (size_t)(4 + slot_bytes(mode)) * ((size_t)1 << b.lg) + 2048 > (size_t)ctx->lds_max)
-        --b.lg;
-      if (b.upper > ((int64_t)1 << b.lg)) b.upper = (int64_t)1 << b.lg;
+      const int64_t cap = (int64_t)(ctx->lds_max - 2048) / (4 + sb);
+      if (b.size > cap) b.size = (int)cap;
+      // every distinct community of a row needs a slot
+      if (b.upper > b.size) b.upper = b.size;
     }
   }
-  // keep bounds increasing after any clamp and end with an unbounded bin
+  // keep bounds increasing after any clamp and end with the unbounded hub bin
   for (int i = 1; i < L.nb; ++i)
     if (L.b[i].upper < L.b[i - 1].upper) L.b[i].upper = L.b[i - 1].upper;
   L.b[L.nb - 1].upper = INT64_MAX;
+  L.b[L.nb - 1].shape = SHAPE_GLOBAL;
   return L;
 }
 
@@ -174,7 +177,10 @@ struct Engine {
   std::vector<GState> hgs;
   // bins of the current level
   std::vector<int64_t> bin_n, bin_off;
-  DBuf btoff, bgkeys, bgvals;
+  // hub plan of the current level (last bin)
+  DBuf hchunk, hv, hbk, hbkhub, htmpc, htmpw, hbktc, hbktw, hbcount, hboff, hbcur, heici, hself,
+      hbgain, hbdst, hgk, hgv, hscan;
+  HubPlan hub;
   // coarsening buffers
   DBuf cflag, crank, cnewc, ck0, ck1, cv0, cv1, chist, chead, cuidx, crowcnt, cscan;
   CoarsenBufs CB{};
@@ -285,23 +291,86 @@ struct Engine {
     HIPC(ctx, hipMemcpyAsync(cursor, cur0.data(), 8 * MGL_MAX_BINS, hipMemcpyHostToDevice, st()));
     HIPC(ctx, hipStreamSynchronize(st()));
     launch_bin_scatter(g.off.as<eid>(), g.n, L, cursor, bperm.as<vid>(), st());
-    // global-hash tables of the last bin
+    // hub plan of the last bin: chunks of each row, hash buckets per hub
+    hub = HubPlan();
     const int last = nb - 1;
     if (L.b[last].shape == SHAPE_GLOBAL && bin_n[last] > 0) {
       const int64_t m = bin_n[last];
-      HIPC(ctx, btoff.ensure(8 * (m + 1)));
-      DBuf sizes;
-      HIPC(ctx, sizes.ensure(8 * m));
-      launch_table_sizes(g.off.as<eid>(), bperm.as<vid>() + bin_off[last], m, sizes.as<int64_t>(),
-                         st());
-      DBuf tmp;
-      HIPC(ctx, tmp.ensure(DevScan::temp_bytes(m + 1)));
-      scan_exclusive_i64(sizes.as<int64_t>(), btoff.as<int64_t>(), m, tmp.p, st());
-      int64_t total = 0;
-      HIPC(ctx, hipMemcpyAsync(&total, btoff.as<int64_t>() + m, 8, hipMemcpyDeviceToHost, st()));
+      const vid* hperm = bperm.as<vid>() + bin_off[last];
+      DBuf rr;
+      HIPC(ctx, rr.ensure(16 * m));
+      launch_row_ranges(g.off.as<eid>(), hperm, m, rr.as<int64_t>(), st());
+      std::vector<int64_t> ranges(2 * m);
+      HIPC(ctx, hipMemcpyAsync(ranges.data(), rr.p, 16 * m, hipMemcpyDeviceToHost, st()));
       HIPC(ctx, hipStreamSynchronize(st()));
-      HIPC(ctx, bgkeys.ensure(4 * total));
-      HIPC(ctx, bgvals.ensure((size_t)slot_bytes(g.mode) * total));
+      const int64_t CH = hub_chunk_edges();
+      const int64_t BK = ctx->hub_bucket;
+      std::vector<int64_t> chunk, hbkv(m + 1, 0), bkhub;
+      int64_t pos = 0;
+      for (int64_t i = 0; i < m; ++i) {
+        const int64_t e0 = ranges[2 * i], e1 = ranges[2 * i + 1], d = e1 - e0;
+        for (int64_t e = e0; e < e1; e += CH) {
+          chunk.push_back(i);
+          chunk.push_back(e);
+          chunk.push_back(pos + (e - e0));
+        }
+        pos += d;
+        int64_t P = (d + BK - 1) / BK;
+        if (P < 1) P = 1;
+        if (P > hub_max_buckets()) P = hub_max_buckets();
+        hbkv[i + 1] = hbkv[i] + P;
+        for (int64_t j = 0; j < P; ++j) bkhub.push_back(i);
+      }
+      hub.m = m;
+      hub.nchunks = (int64_t)chunk.size() / 3;
+      hub.nbuckets = hbkv[m];
+      hub.ehub = pos;
+      const size_t sb = (size_t)slot_bytes(g.mode);
+      const size_t e = (size_t)std::max<int64_t>(pos, 1);
+      HIPC(ctx, hchunk.ensure(8 * std::max<size_t>(chunk.size(), 1)));
+      HIPC(ctx, hv.ensure(4 * m));
+      HIPC(ctx, hbk.ensure(8 * (m + 1)));
+      HIPC(ctx, hbkhub.ensure(8 * hub.nbuckets));
+      HIPC(ctx, htmpc.ensure(4 * e));
+      HIPC(ctx, hbktc.ensure(4 * e));
+      if (g.mode != MODE_UNIT) {
+        HIPC(ctx, htmpw.ensure(sb * e));
+        HIPC(ctx, hbktw.ensure(sb * e));
+      }
+      HIPC(ctx, hbcount.ensure(8 * hub.nbuckets));
+      HIPC(ctx, hboff.ensure(8 * (hub.nbuckets + 1)));
+      HIPC(ctx, hbcur.ensure(8 * hub.nbuckets));
+      HIPC(ctx, heici.ensure(8 * m));
+      HIPC(ctx, hself.ensure(8 * m));
+      HIPC(ctx, hbgain.ensure(8 * hub.nbuckets));
+      HIPC(ctx, hbdst.ensure(4 * hub.nbuckets));
+      HIPC(ctx, hgk.ensure(4 * 2 * e));
+      HIPC(ctx, hgv.ensure(sb * 2 * e));
+      HIPC(ctx, hscan.ensure(DevScan::temp_bytes(hub.nbuckets + 1)));
+      HIPC(ctx, hipMemcpyAsync(hchunk.p, chunk.data(), 8 * chunk.size(), hipMemcpyHostToDevice, st()));
+      HIPC(ctx, hipMemcpyAsync(hv.p, hperm, 4 * m, hipMemcpyDeviceToDevice, st()));
+      HIPC(ctx, hipMemcpyAsync(hbk.p, hbkv.data(), 8 * (m + 1), hipMemcpyHostToDevice, st()));
+      HIPC(ctx, hipMemcpyAsync(hbkhub.p, bkhub.data(), 8 * hub.nbuckets, hipMemcpyHostToDevice, st()));
+      HIPC(ctx, hipStreamSynchronize(st()));
+      hub.chunk = hchunk.as<int64_t>();
+      hub.hub_v = hv.as<vid>();
+      hub.hub_bk = hbk.as<int64_t>();
+      hub.bk_hub = hbkhub.as<int64_t>();
+      hub.tmp_c = htmpc.as<vid>();
+      hub.tmp_w = htmpw.p;
+      hub.bkt_c = hbktc.as<vid>();
+      hub.bkt_w = hbktw.p;
+      hub.bcount = hbcount.as<int64_t>();
+      hub.boff = hboff.as<int64_t>();
+      hub.bcur = hbcur.as<int64_t>();
+      hub.hub_eici = heici.p;
+      hub.hub_self = hself.p;
+      hub.bgain = hbgain.as<double>();
+      hub.bdst = hbdst.as<vid>();
+      hub.gk = hgk.as<vid>();
+      hub.gv = hgv.p;
+      hub.scan_tmp = hscan.p;
+      hub.lds_slots = (uint32_t)((ctx->lds_max - 2048) / (4 + (int)sb) / 16 * 16);
     }
     return MGL_OK;
   }
@@ -331,14 +400,17 @@ struct Engine {
   mgl_status move(const vid* from, vid* to, int up, void* vin, bool time_bins) {
     LevelGraph& g = LG();
     MoveArgs A = move_args(from, to, up, vin);
-    if (timing) A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS;
     int ne = 0;
     for (int b = L.nb - 1; b >= 1; --b) {
       if (bin_n[b] == 0) continue;
+      // per-bin count of aggregated (vertex, neighbour community) pairs
+      if (timing) A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS + b;
       if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b], st()));
-      const int rc = launch_move_bin(g.mode, L.b[b], A, bperm.as<vid>() + bin_off[b], bin_n[b],
-                                     btoff.as<int64_t>(), bgkeys.as<vid>(), bgvals.p, st());
-      if (rc != 0) FAIL(ctx, MGL_UNSUPPORTED, "no move kernel for bin %d (s=%d lg=%d)", b, L.b[b].s, L.b[b].lg);
+      const int rc = (L.b[b].shape == SHAPE_GLOBAL)
+                         ? launch_hub(g.mode, A, hub, st())
+                         : launch_move_bin(g.mode, L.b[b], A, bperm.as<vid>() + bin_off[b],
+                                           bin_n[b], st());
+      if (rc != 0) FAIL(ctx, MGL_UNSUPPORTED, "no move kernel for bin %d (s=%d size=%d)", b, L.b[b].s, L.b[b].size);
       if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b + 1], st()));
       ++ne;
     }
@@ -1076,6 +1148,7 @@ mgl_status mgl_create(mgl_ctx** out, int device) {
     ctx->lds_max = lds;
   (void)hipGetLastError();
   move_kernels_allow_lds(ctx->lds_max);
+  hub_allow_lds(ctx->lds_max);
   (void)hipGetLastError();
   *out = ctx;
   return MGL_OK;
@@ -1184,6 +1257,8 @@ int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins) {
   return L.nb;
 }
 
+int mgl_device_lds_bytes(const mgl_ctx* ctx) { return ctx ? ctx->lds_max : 0; }
+
 }  // extern "C"
 
 // ===========================================================================
@@ -1204,8 +1279,8 @@ __global__ void k_reactivate(GState* gs) {
   s.sin_i = 0; s.sin_d = 0; s.sa2_lo = 0; s.sa2_hi = 0; s.sa2_d = 0;
 }
 __global__ void k_zero_stats(GState* gs, unsigned long long* dist) {
-  gs[0].moved = 0;
-  *dist = 0;
+  if (threadIdx.x == 0) gs[0].moved = 0;
+  dist[threadIdx.x] = 0;
 }
 
 static mgl_status session_prime(mgl_session* s) {
@@ -1288,7 +1363,7 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   HIPC(ctx, hipSetDevice(ctx->device));
   Engine& E = *s->E;
   unsigned long long* dist = E.bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS;
-  k_zero_stats<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>(), dist);
+  k_zero_stats<<<1, MGL_MAX_BINS, 0, ctx->st>>>(E.bgs.as<GState>(), dist);
   E.timing = true;
   s->it += 1;
   mgl_status r = E.iteration(s->it);
@@ -1344,9 +1419,11 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
     for (int32_t v : perm) ents += hoff[v + 1] - hoff[v];
     st.dominant_entries = ents;
   }
-  unsigned long long dh = 0;
-  HIPC(ctx, hipMemcpy(&dh, dist, 8, hipMemcpyDeviceToHost));
-  st.distinct = (int64_t)dh;
+  unsigned long long dh[MGL_MAX_BINS];
+  HIPC(ctx, hipMemcpy(dh, dist, 8 * MGL_MAX_BINS, hipMemcpyDeviceToHost));
+  st.distinct = 0;
+  for (int b = 0; b < MGL_MAX_BINS; ++b) st.distinct += (int64_t)dh[b];
+  if (st.dominant_ms >= 0) st.dominant_distinct = (int64_t)dh[st.dominant_bin];
   st.moved = gs.moved;
   st.q = gs.q_cur;
   // a rejected proposal ends the level; a session keeps passing, so it
@@ -1548,10 +1625,16 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
   char line[256];
   while (fgets(line, sizeof(line), f) && L.nb < MGL_MAX_BINS) {
     if (line[0] == '#' || line[0] == '\n') continue;
+    long long hb;
+    if (sscanf(line, "hub_bucket %lld", &hb) == 1) {
+      if (hb < 1) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_bucket", path);
+      ctx->hub_bucket = hb;
+      continue;
+    }
     long long up;
-    int sh, s, lg;
-    if (sscanf(line, "%lld %d %d %d", &up, &sh, &s, &lg) == 4) {
-      L.b[L.nb++] = {up < 0 ? INT64_MAX : (int64_t)up, sh, s, lg};
+    int sh, s, sz;
+    if (sscanf(line, "%lld %d %d %d", &up, &sh, &s, &sz) == 4) {
+      L.b[L.nb++] = {up < 0 ? INT64_MAX : (int64_t)up, sh, s, sz};
     }
   }
   fclose(f);
@@ -1560,9 +1643,12 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
   for (int i = 1; i < L.nb; ++i) {
     const BinSpec& b = L.b[i];
     if (b.upper <= L.b[i - 1].upper) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bounds", path);
-    if (b.shape == SHAPE_GROUP || b.shape == SHAPE_BLOCK) {
-      // a table of 2^lg slots must hold every distinct community of a row
-      if (b.lg < 1 || b.lg > 15 || b.upper > ((int64_t)1 << b.lg))
+    if (b.shape == SHAPE_GROUP) {
+      // a table of 2^size slots must hold every distinct community of a row
+      if (b.size < 1 || b.size > 15 || b.upper > ((int64_t)1 << b.size))
+        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table too small", path, i);
+    } else if (b.shape == SHAPE_BLOCK) {
+      if (b.size < 1 || b.upper > b.size || (b.s != 256 && b.s != 512 && b.s != 1024))
         FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table too small", path, i);
     } else if (b.shape != SHAPE_GLOBAL) {
       FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d shape", path, i);
@@ -1579,33 +1665,34 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
 // Autotuner: times the level-0 move pass of g under candidate bin layouts
 // (degree bounds, lanes per vertex, LDS table sizes, workgroup sizes) and
 // writes the fastest as a tuning file.
-static std::vector<BinLayout> tuning_candidates() {
+static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
   std::vector<BinLayout> out;
-  out.push_back(default_bins());
-  auto mk = [](std::initializer_list<BinSpec> l) {
+  const BinLayout D = default_bins(ctx->lds_max, 4);
+  out.push_back(D);
+  // variants of the built-in layout: the shapes are kept, the degree bounds
+  // between them move (two entries per lane, earlier workgroups, earlier hubs)
+  auto with = [&](std::initializer_list<BinSpec> head) {
     BinLayout L;
     L.nb = 0;
-    for (const auto& b : l) L.b[L.nb++] = b;
+    for (const auto& b : head) L.b[L.nb++] = b;
+    // append the built-in bins above the last listed bound
+    const int64_t top = L.b[L.nb - 1].upper;
+    for (int i = 0; i < D.nb; ++i)
+      if (D.b[i].upper > top) L.b[L.nb++] = D.b[i];
     return L;
   };
-  // two entries per lane in the sub-wave groups
-  out.push_back(mk({{0, SHAPE_NONE, 0, 0}, {8, SHAPE_GROUP, 4, 4}, {16, SHAPE_GROUP, 8, 5},
-                    {32, SHAPE_GROUP, 16, 6}, {64, SHAPE_GROUP, 32, 7}, {128, SHAPE_GROUP, 64, 8},
-                    {256, SHAPE_GROUP, 64, 9}, {512, SHAPE_GROUP, 64, 10},
-                    {2048, SHAPE_BLOCK, 512, 12}, {4096, SHAPE_BLOCK, 1024, 13},
-                    {INT64_MAX, SHAPE_GLOBAL, 1024, 0}}));
-  // wavefront per vertex from degree 33, larger workgroups earlier
-  out.push_back(mk({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_GROUP, 4, 3}, {8, SHAPE_GROUP, 8, 4},
-                    {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6}, {64, SHAPE_GROUP, 64, 7},
-                    {128, SHAPE_GROUP, 64, 8}, {512, SHAPE_BLOCK, 256, 10},
-                    {2048, SHAPE_BLOCK, 512, 12}, {4096, SHAPE_BLOCK, 1024, 13},
-                    {INT64_MAX, SHAPE_GLOBAL, 1024, 0}}));
-  // denser LDS tables (load factor up to 1)
-  out.push_back(mk({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_GROUP, 4, 3}, {8, SHAPE_GROUP, 8, 4},
-                    {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6}, {64, SHAPE_GROUP, 64, 7},
-                    {128, SHAPE_GROUP, 64, 8}, {512, SHAPE_GROUP, 64, 9},
-                    {2048, SHAPE_BLOCK, 256, 11}, {8192, SHAPE_BLOCK, 1024, 13},
-                    {INT64_MAX, SHAPE_GLOBAL, 1024, 0}}));
+  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {8, SHAPE_GROUP, 4, 4}, {16, SHAPE_GROUP, 8, 5},
+                      {32, SHAPE_GROUP, 16, 6}, {64, SHAPE_GROUP, 32, 7},
+                      {128, SHAPE_GROUP, 64, 8}, {512, SHAPE_GROUP, 64, 10},
+                      {2048, SHAPE_BLOCK, 512, 4096}}));
+  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_GROUP, 4, 3}, {8, SHAPE_GROUP, 8, 4},
+                      {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6},
+                      {64, SHAPE_GROUP, 64, 7}, {128, SHAPE_GROUP, 64, 8},
+                      {512, SHAPE_BLOCK, 256, 1024}, {4096, SHAPE_BLOCK, 1024, 8192}}));
+  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_GROUP, 4, 3}, {8, SHAPE_GROUP, 8, 4},
+                      {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6},
+                      {64, SHAPE_GROUP, 64, 7}, {128, SHAPE_GROUP, 64, 8},
+                      {256, SHAPE_GROUP, 64, 9}, {2048, SHAPE_BLOCK, 512, 4096}}));
   return out;
 }
 
@@ -1615,8 +1702,8 @@ extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* p
   const BinLayout saved = ctx->user_bins;
   const bool had = ctx->have_user_bins;
   double best_ms = 1e300;
-  BinLayout best = default_bins();
-  for (const BinLayout& cand : tuning_candidates()) {
+  BinLayout best = default_bins(ctx->lds_max, 4);
+  for (const BinLayout& cand : tuning_candidates(ctx)) {
     ctx->user_bins = cand;
     ctx->have_user_bins = true;
     mgl_opts o;
@@ -1642,11 +1729,13 @@ extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* p
   ctx->have_user_bins = had;
   FILE* f = fopen(path, "w");
   if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot write %s", path);
-  fprintf(f, "# mgl bin layout: upper shape lanes_or_threads log2_slots (move median %.4f ms)\n",
+  fprintf(f,
+          "# mgl bin layout: upper shape lanes_or_threads size (GROUP: log2 slots, BLOCK: "
+          "slots) -- move median %.4f ms\n",
           best_ms);
   for (int i = 0; i < best.nb; ++i)
     fprintf(f, "%lld %d %d %d\n", best.b[i].upper == INT64_MAX ? -1LL : (long long)best.b[i].upper,
-            best.b[i].shape, best.b[i].s, best.b[i].lg);
+            best.b[i].shape, best.b[i].s, best.b[i].size);
   fclose(f);
   return MGL_OK;
 }

@@ -117,10 +117,30 @@ mgl_status build_csr(std::vector<uint64_t>& keys, int64_t n, int weights, uint64
     for (int64_t i = lo; i < hi; ++i) keys[i] = ((keys[i] >> 32) << b) | (keys[i] & 0xFFFFFFFFull);
   });
   radix_sort_u64(keys, 2 * b);
-  // dedup in place (sequential compaction; keys are sorted)
-  int64_t m = 0;
-  for (int64_t i = 0; i < (int64_t)keys.size(); ++i)
-    if (i == 0 || keys[i] != keys[i - 1]) keys[m++] = keys[i];
+  // dedup (keys are sorted): per-thread compaction, then concatenation
+  const int64_t K = (int64_t)keys.size();
+  const int nt = (K < (1 << 16)) ? 1 : nthreads();
+  std::vector<int64_t> cnt(nt + 1, 0);
+  {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+      th.emplace_back([&, t]() {
+        const int64_t lo = K * t / nt, hi = K * (t + 1) / nt;
+        int64_t o = lo;
+        for (int64_t i = lo; i < hi; ++i)
+          if (i == 0 || keys[i] != keys[i - 1]) keys[o++] = keys[i];
+        cnt[t + 1] = o - lo;
+      });
+    for (auto& x : th) x.join();
+  }
+  for (int t = 0; t < nt; ++t) {
+    const int64_t lo = K * t / nt;
+    if (cnt[t] != lo || t == 0) {
+      if (t > 0) std::memmove(&keys[cnt[t]], &keys[lo], sizeof(uint64_t) * cnt[t + 1]);
+    }
+    cnt[t + 1] += cnt[t];
+  }
+  const int64_t m = cnt[nt];
   keys.resize(m);
   out->n = n;
   out->nnz = m;
@@ -128,9 +148,11 @@ mgl_status build_csr(std::vector<uint64_t>& keys, int64_t n, int weights, uint64
   out->cols = (int32_t*)malloc(sizeof(int32_t) * (m > 0 ? m : 1));
   out->w = weights ? (float*)malloc(sizeof(float) * (m > 0 ? m : 1)) : nullptr;
   if (!out->offsets || !out->cols || (weights && !out->w)) return MGL_OOM;
-  std::fill(out->offsets, out->offsets + n + 1, 0);
-  for (int64_t i = 0; i < m; ++i) out->offsets[(keys[i] >> b) + 1]++;
-  for (int64_t v = 0; v < n; ++v) out->offsets[v + 1] += out->offsets[v];
+  // offsets: the first entry of each row, found by binary search per row
+  pfor(n + 1, [&](int64_t lo, int64_t hi, int) {
+    for (int64_t v = lo; v < hi; ++v)
+      out->offsets[v] = std::lower_bound(keys.begin(), keys.end(), (uint64_t)v << b) - keys.begin();
+  });
   pfor(m, [&](int64_t lo, int64_t hi, int) {
     for (int64_t i = lo; i < hi; ++i) {
       const uint32_t u = (uint32_t)(keys[i] >> b), v = (uint32_t)(keys[i] & vmask);
@@ -177,10 +199,15 @@ mgl_status mgl_gen_rmat(int scale, double ef, double a, double b, double c, uint
       const int64_t lo = m * t / nt, hi = m * (t + 1) / nt;
       auto& P = part[t];
       P.reserve((size_t)(hi - lo) * 2);
+      const uint64_t sk = mix64(seed * 0xD1B54A32D192ED03ull + 0x524D4154ull);
       for (int64_t i = lo; i < hi; ++i) {
         uint64_t u = 0, v = 0;
+        // one 64-bit draw serves two levels (32-bit uniforms)
+        const uint64_t base = mix64(sk ^ ((uint64_t)i * 0x8CB92BA72F3D8DD7ull));
+        uint64_t x = 0;
         for (int l = 0; l < scale; ++l) {
-          const double r = u01(rng(seed, (uint64_t)l, (uint64_t)i));
+          if ((l & 1) == 0) x = mix64(base + (uint64_t)(l >> 1) * 0x9E3779B97F4A7C15ull);
+          const double r = (double)((l & 1) ? (x & 0xFFFFFFFFull) : (x >> 32)) * (1.0 / 4294967296.0);
           u <<= 1;
           v <<= 1;
           if (r < a) {

@@ -1,0 +1,323 @@
+// Hub path of the move pass: rows too long for one workgroup's LDS table.
+//
+// The reference gives each such vertex one thread block and a hash table of
+// 2*max_degree slots in global memory, filled with atomicCAS/atomicAdd per
+// edge (src/louvain/louvain.cu:810-1030, table allocated at :1262-1263).
+// Here a hub's row is split into chunks and processed in five steps, none of
+// which issues a global atomic per edge:
+//   count    : gather c[u] once per entry (kept in a compact buffer), sum the
+//              entries of the own community (eici, self-loop), histogram the
+//              candidates over P buckets (P ~ degree / 4096) by a hash of the id
+//   scan     : bucket offsets
+//   scatter  : counting-sort the candidates into their buckets
+//   aggregate: one workgroup per bucket sums equal ids in an LDS table
+//              (an HBM table if a bucket is unusually large), evaluates the
+//              gain of each distinct id and keeps the bucket's best
+//   finalize : per hub, the best over its buckets, the bi-probing clamp
+// Sums are exact in the integer modes and the argmax is a total order, so the
+// result does not depend on the bucket layout or the order of the scatter.
+#pragma clang fp contract(off)
+
+#include "mgl_kernels.h"
+
+#define HUB_CHUNK 4096
+#define HUB_T 256
+#define HUB_ITEMS (HUB_CHUNK / HUB_T)
+#define HUB_MAXP 1024
+
+__device__ __forceinline__ uint32_t hub_bucket(vid key, uint32_t P) {
+  return (uint32_t)(((uint64_t)((uint32_t)key * 2654435761u) * P) >> 32);
+}
+__device__ __forceinline__ uint32_t hub_slot(vid key, uint32_t H) {
+  uint32_t x = (uint32_t)key * 0x85EBCA6Bu;
+  x ^= x >> 15;
+  x *= 0xC2B2AE35u;
+  return (uint32_t)(((uint64_t)x * H) >> 32);
+}
+
+__device__ __forceinline__ void hatom_add(int32_t* p, int32_t x) { atomicAdd(p, x); }
+__device__ __forceinline__ void hatom_add(int64_t* p, int64_t x) {
+  atomicAdd((unsigned long long*)p, (unsigned long long)x);
+}
+__device__ __forceinline__ void hatom_add(float* p, float x) { atomicAdd(p, x); }
+__device__ __forceinline__ void hatom_add(double* p, double x) { atomicAdd(p, x); }
+
+// table of H slots (any H >= 1), linear probing
+template <class ST, bool GLOBAL>
+__device__ __forceinline__ void tab_insert(vid* K, ST* V, vid key, ST w, uint32_t H) {
+  uint32_t h = hub_slot(key, H);
+  while (true) {
+    vid cur = GLOBAL ? __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                     : __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == MGL_EMPTY) cur = atomicCAS(&K[h], MGL_EMPTY, key);
+    if (cur == MGL_EMPTY || cur == key) { hatom_add(&V[h], w); return; }
+    h = (h + 1u == H) ? 0u : h + 1u;
+  }
+}
+
+template <class T>
+__device__ __forceinline__ T bsum256(T v, T* red) {
+  v = group_sum<64>(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < HUB_T / 64; ++i) s += red[i];
+  return s;
+}
+
+// ---- count ------------------------------------------------------------------
+template <class M>
+__global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
+  typedef typename M::S ST;
+  typedef typename M::K KT;
+  __shared__ unsigned int hist[HUB_MAXP];
+  __shared__ KT red[HUB_T / 64];
+  const int64_t ch = blockIdx.x;
+  const int64_t i = H.chunk[3 * ch];
+  const int64_t e0 = H.chunk[3 * ch + 1];
+  const int64_t pos0 = H.chunk[3 * ch + 2];
+  const vid v = H.hub_v[i];
+  const int g = A.gid ? A.gid[v] : 0;
+  if (!A.gs[g].active) return;
+  const int64_t rend = A.off[v + 1];
+  const int64_t e1 = (e0 + HUB_CHUNK < rend) ? e0 + HUB_CHUNK : rend;
+  const uint32_t P = (uint32_t)(H.hub_bk[i + 1] - H.hub_bk[i]);
+  for (uint32_t b = threadIdx.x; b < P; b += HUB_T) hist[b] = 0;
+  __syncthreads();
+  const vid ci = A.c[v];
+  const bool v1 = A.version == 1;
+  KT eici = 0, selfw = 0;
+  for (int64_t e = e0 + threadIdx.x; e < e1; e += HUB_T) {
+    const vid u = A.col[e];
+    const vid cj = A.c[u];
+    const ST wv = wload<M>(A.w, e);
+    const int64_t pos = pos0 + (e - e0);
+    const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
+    if (cand) {
+      H.tmp_c[pos] = cj;
+      if (M::HAS_W) ((ST*)H.tmp_w)[pos] = wv;
+      atomicAdd(&hist[hub_bucket(cj, P)], 1u);
+    } else {
+      H.tmp_c[pos] = MGL_EMPTY;
+      if (cj == ci) {
+        if (u != v) eici += (KT)wv;
+        else selfw += (KT)wv;
+      }
+    }
+  }
+  eici = bsum256(eici, red);
+  selfw = bsum256(selfw, red);
+  if (threadIdx.x == 0) {
+    if (eici != (KT)0) hatom_add(&((KT*)H.hub_eici)[i], eici);
+    if (selfw != (KT)0) hatom_add(&((KT*)H.hub_self)[i], selfw);
+  }
+  __syncthreads();
+  for (uint32_t b = threadIdx.x; b < P; b += HUB_T)
+    if (hist[b]) atomicAdd((unsigned long long*)&H.bcount[H.hub_bk[i] + b], (unsigned long long)hist[b]);
+}
+
+// ---- scatter ----------------------------------------------------------------
+template <class M>
+__global__ __launch_bounds__(HUB_T) void k_hub_scatter(MoveArgs A, HubPlan H) {
+  typedef typename M::S ST;
+  __shared__ unsigned int cnt[HUB_MAXP];
+  __shared__ int64_t base[HUB_MAXP];
+  const int64_t ch = blockIdx.x;
+  const int64_t i = H.chunk[3 * ch];
+  const int64_t e0 = H.chunk[3 * ch + 1];
+  const int64_t pos0 = H.chunk[3 * ch + 2];
+  const vid v = H.hub_v[i];
+  const int g = A.gid ? A.gid[v] : 0;
+  if (!A.gs[g].active) return;
+  const int64_t rend = A.off[v + 1];
+  const int64_t len = ((e0 + HUB_CHUNK < rend) ? e0 + HUB_CHUNK : rend) - e0;
+  const uint32_t P = (uint32_t)(H.hub_bk[i + 1] - H.hub_bk[i]);
+  for (uint32_t b = threadIdx.x; b < P; b += HUB_T) cnt[b] = 0;
+  __syncthreads();
+  vid key[HUB_ITEMS];
+  unsigned int rk[HUB_ITEMS];
+  uint32_t bk[HUB_ITEMS];
+#pragma unroll
+  for (int j = 0; j < HUB_ITEMS; ++j) {
+    const int64_t r = (int64_t)j * HUB_T + threadIdx.x;
+    key[j] = (r < len) ? H.tmp_c[pos0 + r] : MGL_EMPTY;
+    if (key[j] != MGL_EMPTY) {
+      bk[j] = hub_bucket(key[j], P);
+      rk[j] = atomicAdd(&cnt[bk[j]], 1u);
+    }
+  }
+  __syncthreads();
+  const int64_t bb = H.hub_bk[i];
+  for (uint32_t b = threadIdx.x; b < P; b += HUB_T)
+    if (cnt[b]) base[b] = (int64_t)atomicAdd((unsigned long long*)&H.bcur[bb + b], (unsigned long long)cnt[b]);
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < HUB_ITEMS; ++j) {
+    if (key[j] == MGL_EMPTY) continue;
+    const int64_t dst = base[bk[j]] + rk[j];
+    H.bkt_c[dst] = key[j];
+    if (M::HAS_W) ((ST*)H.bkt_w)[dst] = ((const ST*)H.tmp_w)[pos0 + (int64_t)j * HUB_T + threadIdx.x];
+  }
+}
+
+// ---- aggregate ----------------------------------------------------------------
+template <class M>
+__global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hlds) {
+  typedef typename M::S ST;
+  typedef typename M::K KT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double rg[16];
+  __shared__ vid rd[16];
+  __shared__ int rc[16];
+  const int64_t b = blockIdx.x;
+  const int64_t i = H.bk_hub[b];
+  const vid v = H.hub_v[i];
+  const int g = A.gid ? A.gid[v] : 0;
+  if (!A.gs[g].active) return;
+  const int64_t s0 = H.boff[b], s1 = H.boff[b + 1];
+  const int64_t cnt = s1 - s0;
+  const vid ci = A.c[v];
+  const bool small = A.tie_small ? true : (A.up != 0);
+  double best = 0.0;
+  vid dst = ci;
+  int occ = 0;
+  if (cnt > 0) {
+    const KT ki = ((const KT*)A.k)[v];
+    const KT* a = (const KT*)A.a;
+    const KT aci_k = a[ci] - ki;
+    const double Mh = A.gs[g].mass / 2.0;
+    const double deici = (double)((const KT*)H.hub_eici)[i];
+    const double dki = (double)ki, daci = (double)aci_k;
+    // an LDS table when the bucket fits at load <= 3/4, else an HBM table of
+    // 2*cnt slots at 2*s0 (tables of different buckets do not overlap)
+    const bool lds = (uint64_t)cnt * 4 <= (uint64_t)Hlds * 3;
+    const uint32_t Ht = lds ? Hlds : (uint32_t)(2 * cnt);
+    vid* K;
+    ST* V;
+    if (lds) {
+      K = (vid*)smem;
+      V = (ST*)(smem + (((size_t)4 * Hlds + 15) / 16) * 16);
+    } else {
+      K = H.gk + 2 * s0;
+      V = (ST*)H.gv + 2 * s0;
+    }
+    for (uint32_t s = threadIdx.x; s < Ht; s += 1024) { K[s] = MGL_EMPTY; V[s] = (ST)0; }
+    if (!lds) __threadfence();
+    __syncthreads();
+    for (int64_t j = s0 + threadIdx.x; j < s1; j += 1024) {
+      const vid key = H.bkt_c[j];
+      const ST wv = M::HAS_W ? ((const ST*)H.bkt_w)[j] : (ST)1;
+      if (lds) tab_insert<ST, false>(K, V, key, wv, Ht);
+      else tab_insert<ST, true>(K, V, key, wv, Ht);
+    }
+    if (!lds) __threadfence();
+    __syncthreads();
+    for (uint32_t s = threadIdx.x; s < Ht; s += 1024) {
+      vid cj;
+      ST ev;
+      if (lds) {
+        cj = K[s];
+        ev = V[s];
+      } else {
+        cj = __hip_atomic_load(&K[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ev = (cj != MGL_EMPTY) ? __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (ST)0;
+      }
+      if (cj != MGL_EMPTY) {
+        ++occ;
+        double wij = (double)ev;
+        const double acj = (double)a[cj];
+        wij -= (deici - dki * (daci - acj) / (2. * Mh));  // louvain.cu:540-543
+        wij /= Mh;
+        if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+      }
+    }
+  }
+  // workgroup argmax (cnt is uniform, every thread reaches this point)
+  group_argmax<64>(best, dst, small);
+  occ = group_sum<64>(occ);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { rg[w] = best; rd[w] = dst; rc[w] = occ; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int k2 = 0; k2 < 16; ++k2) {
+      if (better(rg[k2], rd[k2], best, dst, small)) { best = rg[k2]; dst = rd[k2]; }
+      tot += rc[k2];
+    }
+    H.bgain[b] = best;
+    H.bdst[b] = dst;
+    if (A.stats && tot) atomicAdd(A.stats, (unsigned long long)tot);
+  }
+}
+
+// ---- finalize -----------------------------------------------------------------
+template <class M>
+__global__ void k_hub_final(MoveArgs A, HubPlan H) {
+  typedef typename M::K KT;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= H.m) return;
+  const vid v = H.hub_v[i];
+  const int g = A.gid ? A.gid[v] : 0;
+  if (!A.gs[g].active) return;
+  const vid ci = A.c[v];
+  const bool small = A.tie_small ? true : (A.up != 0);
+  double best = 0.0;
+  vid dst = ci;
+  for (int64_t b = H.hub_bk[i]; b < H.hub_bk[i + 1]; ++b)
+    if (better(H.bgain[b], H.bdst[b], best, dst, small)) { best = H.bgain[b]; dst = H.bdst[b]; }
+  A.c_new[v] = (A.version == 1) ? dst : (A.up ? (dst < ci ? dst : ci) : (dst > ci ? dst : ci));
+  if (A.vin) ((KT*)A.vin)[v] = ((const KT*)H.hub_eici)[i] + ((const KT*)H.hub_self)[i];
+}
+
+__global__ void k_hub_copy(const int64_t* s, int64_t* d, int64_t n) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x)
+    d[j] = s[j];
+}
+
+template <class M>
+static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t st) {
+  typedef typename M::S ST;
+  if (H.m <= 0) return 0;
+  launch_fill_bytes0(H.bcount, 8 * (size_t)H.nbuckets, st);
+  launch_fill_bytes0(H.hub_eici, 8 * (size_t)H.m, st);
+  launch_fill_bytes0(H.hub_self, 8 * (size_t)H.m, st);
+  k_hub_count<M><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+  scan_exclusive_i64(H.bcount, H.boff, H.nbuckets, H.scan_tmp, st);
+  {
+    int64_t gr = (H.nbuckets + 255) / 256;
+    if (gr > 4096) gr = 4096;
+    k_hub_copy<<<(unsigned)gr, 256, 0, st>>>(H.boff, H.bcur, H.nbuckets);
+  }
+  k_hub_scatter<M><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+  const size_t lds = (((size_t)4 * Hlds + 15) / 16) * 16 + sizeof(ST) * (size_t)Hlds;
+  k_hub_aggregate<M><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+  k_hub_final<M><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
+  return 0;
+}
+
+int launch_hub(int mode, const MoveArgs& A, const HubPlan& H, hipStream_t st) {
+  switch (mode) {
+    case MODE_UNIT: return hub_t<ModeUnit>(A, H, H.lds_slots, st);
+    case MODE_INT: return hub_t<ModeInt>(A, H, H.lds_slots, st);
+    case MODE_F64: return hub_t<ModeF64>(A, H, H.lds_slots, st);
+    case MODE_F32: return hub_t<ModeF32>(A, H, H.lds_slots, st);
+  }
+  return -1;
+}
+
+void hub_allow_lds(int bytes) {
+  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<ModeUnit>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<ModeInt>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<ModeF64>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<ModeF32>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+int hub_chunk_edges() { return HUB_CHUNK; }
+int hub_max_buckets() { return HUB_MAXP; }

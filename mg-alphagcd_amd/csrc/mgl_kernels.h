@@ -6,16 +6,16 @@
 // A bin holds vertices whose degree lies in (upper[b-1], upper[b]]. Bin 0 is
 // degree 0 (isolated vertices never move and are never launched). Each bin is
 // served by one kernel shape:
-//   SHAPE_GROUP : S lanes of a wavefront per vertex, LDS hash of H slots each
-//   SHAPE_BLOCK : one workgroup of BS threads per vertex, LDS hash of H slots
-//   SHAPE_GLOBAL: one workgroup of 1024 threads per vertex, hash in HBM sized
-//                 2*nextpow2(degree) per vertex
+//   SHAPE_GROUP : S lanes of a wavefront per vertex, LDS hash of 2^lg slots each
+//   SHAPE_BLOCK : one workgroup of BS threads per vertex, LDS hash of `size` slots
+//   SHAPE_GLOBAL: hub path (mgl_hub.hip): rows split into chunks, candidates
+//                 bucketed by hash, one workgroup per bucket
 enum Shape { SHAPE_NONE = 0, SHAPE_GROUP = 1, SHAPE_BLOCK = 2, SHAPE_GLOBAL = 3 };
 struct BinSpec {
   int64_t upper;  // inclusive degree bound (INT64_MAX for the last bin)
   int shape;
-  int s;          // lanes per vertex (GROUP) or threads per block (BLOCK/GLOBAL)
-  int lg;         // log2 of hash slots per vertex (GROUP/BLOCK)
+  int s;          // lanes per vertex (GROUP) or threads per block (BLOCK)
+  int size;       // GROUP: log2 of slots per vertex; BLOCK: slots per vertex
 };
 #define MGL_MAX_BINS 16
 
@@ -23,7 +23,9 @@ struct BinLayout {
   int nb = 0;
   BinSpec b[MGL_MAX_BINS];
 };
-BinLayout default_bins();
+// Built-in layout for a device whose workgroups may use lds_bytes of LDS and a
+// mode whose hash slots take slot_bytes (+4 for the key).
+BinLayout default_bins(int lds_bytes, int slot_bytes);
 
 // ---- move ------------------------------------------------------------------
 struct MoveArgs {
@@ -43,14 +45,40 @@ struct MoveArgs {
   unsigned long long* stats; // [0] distinct pairs (nullable)
 };
 
-// Launch the move kernel of one bin. perm points at the bin's vertices.
-// For SHAPE_GLOBAL, toff[i] is the slot offset of the i-th vertex's table
-// (toff[nb] = total) and keys/vals point at scratch of that many slots.
-// Returns 0, or -1 when no kernel instance matches the bin's shape.
+// Launch the move kernel of one GROUP or BLOCK bin. perm points at the bin's
+// vertices. Returns 0, or -1 when no kernel instance matches the bin's shape.
 int launch_move_bin(int mode, const BinSpec& bs, const MoveArgs& A, const vid* perm,
-                    int64_t nb, const int64_t* toff, vid* gkeys, void* gvals,
-                    hipStream_t st);
+                    int64_t nb, hipStream_t st);
 void move_kernels_allow_lds(int bytes);
+
+// ---- hub path (mgl_hub.hip) ------------------------------------------------
+struct HubPlan {
+  int64_t m = 0, nchunks = 0, nbuckets = 0, ehub = 0;
+  const int64_t* chunk = nullptr;   // [3*nchunks]: hub index, first CSR entry, compact position
+  const vid* hub_v = nullptr;       // [m]
+  const int64_t* hub_bk = nullptr;  // [m+1] bucket ranges
+  const int64_t* bk_hub = nullptr;  // [nbuckets] owning hub
+  vid* tmp_c = nullptr;             // [ehub] neighbour community or EMPTY
+  void* tmp_w = nullptr;            // [ehub] weights (weighted modes)
+  vid* bkt_c = nullptr;             // [ehub] bucketed candidates
+  void* bkt_w = nullptr;
+  int64_t* bcount = nullptr;        // [nbuckets]
+  int64_t* boff = nullptr;          // [nbuckets+1]
+  int64_t* bcur = nullptr;          // [nbuckets]
+  void* hub_eici = nullptr;         // K[m]
+  void* hub_self = nullptr;         // K[m]
+  double* bgain = nullptr;          // [nbuckets]
+  vid* bdst = nullptr;              // [nbuckets]
+  vid* gk = nullptr;                // [2*ehub] HBM tables for oversized buckets
+  void* gv = nullptr;
+  void* scan_tmp = nullptr;
+  uint32_t lds_slots = 0;           // LDS table slots of the aggregate kernel
+};
+int launch_hub(int mode, const MoveArgs& A, const HubPlan& H, hipStream_t st);
+void hub_allow_lds(int bytes);
+int hub_chunk_edges();
+int hub_max_buckets();
+void launch_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out, hipStream_t st);
 
 // ---- per-vertex row reductions (vertex weights, modularity sum_in) ----------
 struct RowArgs {
@@ -73,8 +101,6 @@ void launch_bin_count(const eid* off, int64_t n, const BinLayout& L, unsigned lo
                       hipStream_t st);
 void launch_bin_scatter(const eid* off, int64_t n, const BinLayout& L,
                         unsigned long long* cursor /* [nb] starts at bin offsets */, vid* perm,
-                        hipStream_t st);
-void launch_table_sizes(const eid* off, const vid* perm, int64_t nb, int64_t* sizes,
                         hipStream_t st);
 
 // ---- community-weight update, reductions, control --------------------------

@@ -183,13 +183,21 @@ __device__ __forceinline__ void block_argmax(double& bg, vid& bd, bool small, do
     if (better(rg[i], rd[i], bg, bd, small)) { bg = rg[i]; bd = rd[i]; }
 }
 
-// SHAPE_BLOCK / SHAPE_GLOBAL: one workgroup per vertex. GLOBAL keeps the
-// table in HBM (keys gk, values gv at toff[i]); BLOCK in dynamic LDS.
-template <class M, int BS, bool GLOBAL>
+// SHAPE_BLOCK: one workgroup per vertex, an LDS table of H slots (any H).
+template <class ST>
+__device__ __forceinline__ void lds_insert_any(vid* K, ST* V, vid key, ST w, uint32_t H) {
+  uint32_t h = any_slot(key, H);
+  while (true) {
+    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == MGL_EMPTY) cur = atomicCAS(&K[h], MGL_EMPTY, key);
+    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
+    h = (h + 1u == H) ? 0u : h + 1u;
+  }
+}
+
+template <class M, int BS>
 __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __restrict__ perm,
-                                                   int64_t nb, int lg_lds,
-                                                   const int64_t* __restrict__ toff, vid* gk,
-                                                   typename M::S* gv) {
+                                                   int64_t nb, uint32_t H) {
   typedef typename M::S ST;
   typedef typename M::K KT;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -202,21 +210,9 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   const vid v = perm[gi];
   int g;
   if (!vertex_active<M>(A, v, g)) return;  // uniform over the workgroup
-  int lg;
-  vid* K;
-  ST* V;
-  if (GLOBAL) {
-    const int64_t H = toff[gi + 1] - toff[gi];
-    lg = 63 - __builtin_clzll((unsigned long long)H);
-    K = gk + toff[gi];
-    V = gv + toff[gi];
-  } else {
-    lg = lg_lds;
-    K = (vid*)smem;
-    V = (ST*)(smem + (((size_t)4 << lg) + 15) / 16 * 16);
-  }
-  const int H = 1 << lg;
-  for (int i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  vid* K = (vid*)smem;
+  ST* V = (ST*)(smem + (((size_t)4 * H + 15) / 16) * 16);
+  for (uint32_t i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
   __syncthreads();
   const vid ci = A.c[v];
   const eid e0 = A.off[v], e1 = A.off[v + 1];
@@ -227,14 +223,11 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
     const vid cj = A.c[u];
     const ST wv = wload<M>(A.w, e);
     const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
-    if (cand) {
-      if (GLOBAL) glb_insert<ST>(K, V, cj, wv, lg);
-      else lds_insert<ST>(K, V, cj, wv, lg);
-    } else if (cj == ci) {
+    if (cand) lds_insert_any<ST>(K, V, cj, wv, H);
+    else if (cj == ci) {
       if (u != v) eici += wv; else selfw += wv;
     }
   }
-  if (GLOBAL) __threadfence();
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
   selfw = block_sum<BS>(selfw, rs);
@@ -247,19 +240,11 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   double best = 0.0;
   vid dst = ci;
   int cnt = 0;
-  for (int i = threadIdx.x; i < H; i += BS) {
-    vid cj;
-    ST ev;
-    if (GLOBAL) {
-      cj = __hip_atomic_load(&K[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      ev = (cj != MGL_EMPTY) ? __hip_atomic_load(&V[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (ST)0;
-    } else {
-      cj = K[i];
-      ev = V[i];
-    }
+  for (uint32_t i = threadIdx.x; i < H; i += BS) {
+    const vid cj = K[i];
     if (cj != MGL_EMPTY) {
       ++cnt;
-      const double wij = gain((double)ev, deici, dki, daci, (double)a[cj], Mh);
+      const double wij = gain((double)V[i], deici, dki, daci, (double)a[cj], Mh);
       if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
     }
   }
@@ -273,7 +258,7 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
 }
 
 // ---------------------------------------------------------------------------
-BinLayout default_bins() {
+BinLayout default_bins(int lds_bytes, int slot_bytes) {
   BinLayout L;
   int i = 0;
   L.b[i++] = {0, SHAPE_NONE, 0, 0};
@@ -284,23 +269,26 @@ BinLayout default_bins() {
   L.b[i++] = {64, SHAPE_GROUP, 64, 7};
   L.b[i++] = {128, SHAPE_GROUP, 64, 8};
   L.b[i++] = {256, SHAPE_GROUP, 64, 9};
-  L.b[i++] = {1024, SHAPE_BLOCK, 256, 11};
-  L.b[i++] = {4096, SHAPE_BLOCK, 1024, 13};
-  L.b[i++] = {INT64_MAX, SHAPE_GLOBAL, 1024, 0};
+  L.b[i++] = {1024, SHAPE_BLOCK, 256, 2048};
+  L.b[i++] = {4096, SHAPE_BLOCK, 1024, 8192};
+  // the largest table the workgroup LDS holds, filled to at most 3/4
+  const int64_t hmax = (int64_t)(lds_bytes - 2048) / (4 + slot_bytes) / 16 * 16;
+  if (hmax * 3 / 4 > 4096) L.b[i++] = {hmax * 3 / 4, SHAPE_BLOCK, 1024, (int)hmax};
+  L.b[i++] = {INT64_MAX, SHAPE_GLOBAL, 0, 0};
   L.nb = i;
   return L;
 }
 
 template <class M>
 static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int64_t nb,
-                      const int64_t* toff, vid* gk, void* gv, hipStream_t st) {
+                      hipStream_t st) {
   typedef typename M::S ST;
   if (nb <= 0 || bs.shape == SHAPE_NONE) return 0;
   if (bs.shape == SHAPE_GROUP) {
     const int gpb = 256 / bs.s;
     const unsigned grid = (unsigned)((nb + gpb - 1) / gpb);
 #define G_(S_, LG_)                                                                     \
-  if (bs.s == S_ && bs.lg == LG_) {                                                     \
+  if (bs.s == S_ && bs.size == LG_) {                                                   \
     k_move_group<M, S_, LG_><<<grid, 256, 0, st>>>(A, perm, nb);                        \
     return 0;                                                                           \
   }
@@ -308,37 +296,31 @@ static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int
     G_(64, 7) G_(64, 8) G_(64, 9) G_(64, 10)
 #undef G_
   } else if (bs.shape == SHAPE_BLOCK) {
-    const size_t lds = (((size_t)4 << bs.lg) + 15) / 16 * 16 + (sizeof(ST) << bs.lg);
+    const uint32_t H = (uint32_t)bs.size;
+    const size_t lds = (((size_t)4 * H + 15) / 16) * 16 + sizeof(ST) * (size_t)H;
     if (bs.s == 256) {
-      k_move_block<M, 256, false><<<(unsigned)nb, 256, lds, st>>>(A, perm, nb, bs.lg, nullptr,
-                                                                   nullptr, nullptr);
+      k_move_block<M, 256><<<(unsigned)nb, 256, lds, st>>>(A, perm, nb, H);
       return 0;
     }
     if (bs.s == 512) {
-      k_move_block<M, 512, false><<<(unsigned)nb, 512, lds, st>>>(A, perm, nb, bs.lg, nullptr,
-                                                                   nullptr, nullptr);
+      k_move_block<M, 512><<<(unsigned)nb, 512, lds, st>>>(A, perm, nb, H);
       return 0;
     }
     if (bs.s == 1024) {
-      k_move_block<M, 1024, false><<<(unsigned)nb, 1024, lds, st>>>(A, perm, nb, bs.lg,
-                                                                     nullptr, nullptr, nullptr);
+      k_move_block<M, 1024><<<(unsigned)nb, 1024, lds, st>>>(A, perm, nb, H);
       return 0;
     }
-  } else if (bs.shape == SHAPE_GLOBAL) {
-    k_move_block<M, 1024, true><<<(unsigned)nb, 1024, 0, st>>>(A, perm, nb, 0, toff, gk,
-                                                                (ST*)gv);
-    return 0;
   }
   return -1;
 }
 
 int launch_move_bin(int mode, const BinSpec& bs, const MoveArgs& A, const vid* perm,
-                    int64_t nb, const int64_t* toff, vid* gk, void* gv, hipStream_t st) {
+                    int64_t nb, hipStream_t st) {
   switch (mode) {
-    case MODE_UNIT: return move_bin_t<ModeUnit>(bs, A, perm, nb, toff, gk, gv, st);
-    case MODE_INT: return move_bin_t<ModeInt>(bs, A, perm, nb, toff, gk, gv, st);
-    case MODE_F64: return move_bin_t<ModeF64>(bs, A, perm, nb, toff, gk, gv, st);
-    case MODE_F32: return move_bin_t<ModeF32>(bs, A, perm, nb, toff, gk, gv, st);
+    case MODE_UNIT: return move_bin_t<ModeUnit>(bs, A, perm, nb, st);
+    case MODE_INT: return move_bin_t<ModeInt>(bs, A, perm, nb, st);
+    case MODE_F64: return move_bin_t<ModeF64>(bs, A, perm, nb, st);
+    case MODE_F32: return move_bin_t<ModeF32>(bs, A, perm, nb, st);
   }
   return -1;
 }
@@ -346,7 +328,7 @@ int launch_move_bin(int mode, const BinSpec& bs, const MoveArgs& A, const vid* p
 // Opt the LDS-table kernels into the device's largest workgroup LDS size.
 void move_kernels_allow_lds(int bytes) {
 #define A_(M_, BS_) \
-  (void)hipFuncSetAttribute((const void*)k_move_block<M_, BS_, false>, \
+  (void)hipFuncSetAttribute((const void*)k_move_block<M_, BS_>, \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   A_(ModeUnit, 256) A_(ModeUnit, 512) A_(ModeUnit, 1024)
   A_(ModeInt, 256) A_(ModeInt, 512) A_(ModeInt, 1024)
@@ -512,20 +494,18 @@ void launch_bin_scatter(const eid* off, int64_t n, const BinLayout& L, unsigned 
   k_bin_scatter<<<grid_for(n, 256, 4096), 256, 0, st>>>(off, n, bounds_of(L), cursor, perm);
 }
 
-// hash-table slots of a global-hash vertex: 2 * nextpow2(degree), at least 64
-__global__ void k_table_sizes(const eid* off, const vid* perm, int64_t nb, int64_t* sizes) {
+// (first entry, end) of the rows of perm[0..nb)
+__global__ void k_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nb) return;
   const vid v = perm[i];
-  int64_t d = off[v + 1] - off[v];
-  int64_t p = 64;
-  while (p < 2 * d) p <<= 1;
-  sizes[i] = p;
+  out[2 * i] = off[v];
+  out[2 * i + 1] = off[v + 1];
 }
-void launch_table_sizes(const eid* off, const vid* perm, int64_t nb, int64_t* sizes,
-                        hipStream_t st) {
+void launch_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out,
+                       hipStream_t st) {
   if (nb <= 0) return;
-  k_table_sizes<<<grid_for(nb, 256, 1u << 30), 256, 0, st>>>(off, perm, nb, sizes);
+  k_row_ranges<<<grid_for(nb, 256, 1u << 30), 256, 0, st>>>(off, perm, nb, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -104,19 +104,59 @@ def test_coarsen_stage(ctx, mgl, orc):
     s.close()
 
 
-def test_hub_uses_global_bin(ctx, mgl, orc):
-    # a star with 9000 leaves plus a ring: the centre's degree exceeds the
-    # LDS-table bins, so its row goes through the HBM hash
-    n = 9001
+def test_hub_path_star(ctx, mgl, orc):
+    # a star with 40000 leaves plus a ring: the centre's degree exceeds every
+    # LDS-table bin, so its row goes through the bucketed hub path
+    n = 40001
     edges = [(0, i) for i in range(1, n)] + [(i, i + 1) for i in range(1, n - 1)]
     g = mgl.CSR.from_edges(n, edges)
+    s = ctx.session(g)
+    assert s.bins()[-1] == 1
+    s.close()
     res = ctx.louvain(g, keep_levels=True, keep_graphs=True)
     ref = orc.louvain(g.offsets, g.cols, None)
     compare(res, ref)
-    s = ctx.session(g)
+
+
+def hub_graph(mgl):
+    # R-MAT with a heavy tail: many rows in the hub bin
+    return mgl.gen_rmat(17, 48, seed=13)
+
+
+@pytest.mark.parametrize("bucket", [4096, 1 << 20])
+def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, bucket):
+    # bucket = 2^20 makes each hub a single bucket larger than the LDS table,
+    # which exercises the HBM-table fallback of the aggregate step
+    g = hub_graph(mgl)
+    tf = tmp_path / "tune.txt"
+    tf.write_text(f"hub_bucket {bucket}\n" + "".join(
+        f"{u} {sh} {s_} {sz}\n" for u, sh, s_, sz in
+        [(0, 0, 0, 0), (4, 1, 4, 3), (8, 1, 8, 4), (16, 1, 16, 5), (32, 1, 32, 6),
+         (64, 1, 64, 7), (128, 1, 64, 8), (256, 1, 64, 9), (1024, 2, 256, 2048),
+         (-1, 3, 0, 0)]))
+    s = ctx.session(g, tuning_file=str(tf))
     bins = s.bins()
-    assert bins[-1] == 1
+    print("bins", bins, flush=True)
+    assert bins[-1] > 10
+    k = orc.vertex_weights(g.offsets)
+    rng = np.random.default_rng(5)
+    for trial in range(3):
+        c = rng.integers(0, g.n, g.n).astype(np.int32)
+        if trial > 0:
+            c = (c // (7 * trial)) * (7 * trial)
+        s.set_assign(c)
+        a = orc.community_totals(k, c, g.n)
+        for up in (True, False):
+            np.testing.assert_array_equal(
+                s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
     s.close()
+
+
+def test_hub_graph_full_parity(ctx, mgl, orc):
+    g = hub_graph(mgl)
+    res = ctx.louvain(g, keep_levels=True, keep_graphs=True)
+    ref = orc.louvain(g.offsets, g.cols, None)
+    compare(res, ref)
 
 
 def test_edge_cases(ctx, mgl, orc):
