@@ -73,6 +73,7 @@ typedef struct {
   int keep_graphs;     /* 1: keep per-level coarse CSRs in the result */
   int validate;        /* 1 (default): check offsets, column range and symmetry */
   const char* tuning_file; /* bin thresholds written by mgl_autotune, or NULL for built-in defaults */
+  int use_graph;       /* 1 (default): the inner loop of a level is captured once as a hipGraph and replayed */
 } mgl_opts;
 
 typedef struct mgl_ctx mgl_ctx;

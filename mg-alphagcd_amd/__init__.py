@@ -60,7 +60,8 @@ class _Opts(C.Structure):
     _fields_ = [("version", C.c_int), ("threshold", C.c_double), ("max_phases", C.c_int),
                 ("max_iter", C.c_int), ("precision", C.c_int), ("tie_rule", C.c_int),
                 ("fuse_modularity", C.c_int), ("keep_levels", C.c_int),
-                ("keep_graphs", C.c_int), ("validate", C.c_int), ("tuning_file", C.c_char_p)]
+                ("keep_graphs", C.c_int), ("validate", C.c_int), ("tuning_file", C.c_char_p),
+                ("use_graph", C.c_int)]
 
 
 class PassStats(C.Structure):
@@ -273,7 +274,7 @@ def _read_result(r) -> Result:
 
 def make_opts(version=2, threshold=1e-6, max_phases=1000, max_iter=0, precision=PREC_AUTO,
               tie_rule=TIE_REFERENCE, fuse_modularity=True, keep_levels=False,
-              keep_graphs=False, validate=1, tuning_file=None) -> _Opts:
+              keep_graphs=False, validate=1, tuning_file=None, use_graph=True) -> _Opts:
     o = _Opts()
     _lib.mgl_opts_default(C.byref(o))
     o.version, o.threshold, o.max_phases, o.max_iter = version, threshold, max_phases, max_iter
@@ -281,6 +282,7 @@ def make_opts(version=2, threshold=1e-6, max_phases=1000, max_iter=0, precision=
     o.fuse_modularity, o.keep_levels, o.keep_graphs = int(fuse_modularity), int(keep_levels), int(keep_graphs)
     o.validate = int(validate)
     o.tuning_file = tuning_file.encode() if tuning_file else None
+    o.use_graph = int(use_graph)
     return o
 
 

@@ -205,6 +205,8 @@ struct Engine {
       if (e) (void)hipEventDestroy(e);
   }
   hipStream_t st() const { return ctx->st; }
+  // device word: graphs still active after the last decision
+  int* nactive() { return (int*)(bcounts.as<unsigned long long>() + 3 * MGL_MAX_BINS); }
   LevelGraph& LG() { return lg[cur]; }
 
   mgl_status alloc_state() {
@@ -225,7 +227,7 @@ struct Engine {
     HIPC(ctx, bdelta.ensure(8 * G));
     HIPC(ctx, bgather.ensure(8 * (G + 1) * 2));
     HIPC(ctx, btraj.ensure(sizeof(double) * (size_t)G * TRAJ_MAX));
-    HIPC(ctx, bcounts.ensure(8 * MGL_MAX_BINS * 3));
+    HIPC(ctx, bcounts.ensure(8 * MGL_MAX_BINS * 4));
     c = bc.as<vid>(); p = bp.as<vid>(); nx = bn.as<vid>();
     if (!ev_ok) {
       for (auto& x : ev) HIPC(ctx, hipEventCreate(&x));
@@ -491,7 +493,8 @@ struct Engine {
       launch_update(g.mode, g.n, p, nx, bk.p, ba.p, gid, gs, st());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 2], st()));
       launch_reduce(g.mode, g.n, bvin.p, ba.p, gid, gs, st());
-      launch_decide(G, gs, o.threshold, o.max_iter, 1, ex, traj, TRAJ_MAX, 0, st());
+      launch_fill_bytes0(nactive(), 4, st());
+      launch_decide(G, gs, o.threshold, o.max_iter, 1, ex, traj, TRAJ_MAX, 0, st(), nactive());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 3], st()));
       launch_fixup(g.n, c, p, nx, gid, gs, st());
       vid* t = c; c = p; p = nx; nx = t;
@@ -505,7 +508,8 @@ struct Engine {
       s = rows(OP_INTRA, p, nullptr, bvin.p, true);
       if (s) return s;
       launch_reduce(g.mode, g.n, bvin.p, ba.p, gid, gs, st());
-      launch_decide(G, gs, o.threshold, o.max_iter, 0, ex, traj, TRAJ_MAX, 0, st());
+      launch_fill_bytes0(nactive(), 4, st());
+      launch_decide(G, gs, o.threshold, o.max_iter, 0, ex, traj, TRAJ_MAX, 0, st(), nactive());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 3], st()));
       launch_fixup(g.n, c, p, nullptr, gid, gs, st());
     }
@@ -529,23 +533,75 @@ struct Engine {
     return MGL_OK;
   }
 
-  // Inner loop of a level (louvain.cu:1548-1660).
+  mgl_status read_nactive(int& na) {
+    HIPC(ctx, hipMemcpyAsync(&na, nactive(), 4, hipMemcpyDeviceToHost, st()));
+    HIPC(ctx, hipStreamSynchronize(st()));
+    return MGL_OK;
+  }
+
+  // Inner loop of a level (louvain.cu:1548-1660). With use_graph the
+  // iterations are captured once per level into a hipGraph holding R of them
+  // (R = 6: the proposal buffers rotate with period 3 and the up/down
+  // direction alternates with period 2) and replayed until no graph of the
+  // batch is active; iterations after a graph's last decision are no-ops for it.
   mgl_status run_level() {
     mgl_status s;
     if (o.fuse_modularity) {
       s = pre_move();
       if (s) return s;
     }
-    for (int it = 1;; ++it) {
-      if (it > (1 << 22)) FAIL(ctx, MGL_DEVICE_ERROR, "inner loop did not terminate");
-      s = iteration(it);
-      if (s) return s;
-      bool any;
-      s = any_active(any);
-      if (s) return s;
-      if (!any) break;
+    if (!o.use_graph) {
+      for (int it = 1;; ++it) {
+        if (it > (1 << 22)) FAIL(ctx, MGL_DEVICE_ERROR, "inner loop did not terminate");
+        s = iteration(it);
+        if (s) return s;
+        int na;
+        s = read_nactive(na);
+        if (s) return s;
+        if (na == 0) break;
+      }
+      return MGL_OK;
     }
-    return MGL_OK;
+    const int R = o.fuse_modularity ? 6 : 2;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPC(ctx, hipStreamBeginCapture(st(), hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < R; ++j) {
+      s = iteration(1 + j);
+      if (s) {
+        (void)hipStreamEndCapture(st(), &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return s;
+      }
+    }
+    HIPC(ctx, hipStreamEndCapture(st(), &graph));
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      HIPC(ctx, e);
+    }
+    mgl_status rs = MGL_OK;
+    for (int rep = 0;; ++rep) {
+      if ((int64_t)rep * R > (1 << 22)) {
+        ctx->err = "inner loop did not terminate";
+        rs = MGL_DEVICE_ERROR;
+        break;
+      }
+      e = hipGraphLaunch(exec, st());
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        ctx->err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+        rs = MGL_DEVICE_ERROR;
+        break;
+      }
+      int na;
+      rs = read_nactive(na);
+      if (rs || na == 0) break;
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    return rs;
   }
 
   mgl_status level_end(int level);
@@ -1119,6 +1175,7 @@ void mgl_opts_default(mgl_opts* o) {
   o->keep_graphs = 0;
   o->validate = 1;
   o->tuning_file = nullptr;
+  o->use_graph = 1;
 }
 
 int mgl_version_major(void) { return 1; }

@@ -113,8 +113,9 @@ void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const in
 // singleton reduction, q_cur = Q(singletons).
 void launch_level_begin(int ng, GState* gs, const int32_t* run, hipStream_t st);
 void launch_level_init(int ng, GState* gs, int exact, hipStream_t st);
+// nactive (optional) receives the number of graphs still active afterwards.
 void launch_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
-                   double* traj, int traj_max, int pre, hipStream_t st);
+                   double* traj, int traj_max, int pre, hipStream_t st, int* nactive = nullptr);
 void launch_fixup(int64_t n, vid* c, vid* p, vid* nx, const int32_t* gid, const GState* gs,
                   hipStream_t st);
 void launch_iota(vid* x, int64_t n, hipStream_t st);

@@ -709,12 +709,19 @@ void launch_level_init(int ng, GState* gs, int exact, hipStream_t st) {
 
 // pre = 1: after the first move of a level (fused mode) only stores sum a^2 of
 // the first proposal; no decision is taken.
+__device__ void decide_one(GState& s, int g, double thr, int max_iter, int fused, int exact,
+                           double* traj, int traj_max, int pre);
 __global__ void k_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
-                         double* traj, int traj_max, int pre) {
+                         double* traj, int traj_max, int pre, int* nactive) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   GState& s = gs[g];
   if (!s.active) { s.fix = 0; reset_acc(s); return; }
+  decide_one(s, g, thr, max_iter, fused, exact, traj, traj_max, pre);
+  if (nactive && s.active) atomicAdd(nactive, 1);
+}
+__device__ void decide_one(GState& s, int g, double thr, int max_iter, int fused, int exact,
+                           double* traj, int traj_max, int pre) {
   if (pre) {
     s.sa2p_lo = s.sa2_lo; s.sa2p_hi = s.sa2_hi; s.sa2p_d = s.sa2_d;
     s.fix = 0;
@@ -738,10 +745,11 @@ __global__ void k_decide(int ng, GState* gs, double thr, int max_iter, int fused
   if (fused) { s.sa2p_lo = s.sa2_lo; s.sa2p_hi = s.sa2_hi; s.sa2p_d = s.sa2_d; }
   reset_acc(s);
 }
+
 void launch_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
-                   double* traj, int traj_max, int pre, hipStream_t st) {
+                   double* traj, int traj_max, int pre, hipStream_t st, int* nactive) {
   k_decide<<<(ng + 255) / 256, 256, 0, st>>>(ng, gs, thr, max_iter, fused, exact, traj, traj_max,
-                                             pre);
+                                             pre, nactive);
 }
 
 // Buffer fix-up after a decision:

@@ -58,11 +58,12 @@ def test_two_triangles_known_answer(ctx, mgl, version):
 
 @pytest.mark.parametrize("version", [0, 1, 2])
 @pytest.mark.parametrize("fused", [True, False])
-def test_rmat16_full_parity(ctx, mgl, orc, version, fused):
+@pytest.mark.parametrize("graph", [True, False])
+def test_rmat16_full_parity(ctx, mgl, orc, version, fused, graph):
     g = mgl.gen_rmat(16, 16, seed=1)
-    print(f"rmat16 v{version} fused={fused} n={g.n} nnz={g.nnz}", flush=True)
-    res = ctx.louvain(g, version=version, fuse_modularity=fused, keep_levels=True,
-                      keep_graphs=True)
+    print(f"rmat16 v{version} fused={fused} graph={graph} n={g.n} nnz={g.nnz}", flush=True)
+    res = ctx.louvain(g, version=version, fuse_modularity=fused, use_graph=graph,
+                      keep_levels=True, keep_graphs=True)
     ref = orc.louvain(g.offsets, g.cols, None, mode=0, version=version)
     compare(res, ref)
 
@@ -189,13 +190,14 @@ def test_sbm_weighted_exact(ctx, mgl, orc):
     compare(res, ref, scale=S)
 
 
-def test_batched_equals_plain(ctx, mgl):
+@pytest.mark.parametrize("graph", [True, False])
+def test_batched_equals_plain(ctx, mgl, graph):
     graphs = [mgl.gen_rmat(10 + (i % 3), 8 + 4 * (i % 3), seed=100 + i) for i in range(5)]
     graphs.append(two_triangles(mgl))
     graphs.append(mgl.gen_sbm(3000, 12, min_size=64, seed=11, weights=0, deg_in=16, deg_out=3))
-    batch = ctx.louvain_batched(graphs, keep_levels=True)
+    batch = ctx.louvain_batched(graphs, keep_levels=True, use_graph=graph)
     for g, b in zip(graphs, batch):
-        p = ctx.louvain(g, keep_levels=True)
+        p = ctx.louvain(g, keep_levels=True, use_graph=False)
         assert len(b.levels) == len(p.levels)
         for lb, lp in zip(b.levels, p.levels):
             assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
