@@ -41,6 +41,14 @@ __device__ __forceinline__ typename M::S wload(const void* w, eid e) {
   else return (typename M::S)1;
 }
 
+// Per-vertex record gathered once per CSR entry: the vertex's community and
+// that community's total weight.
+template <class AT>
+struct __align__(sizeof(AT) >= 8 ? 16 : 8) Rec {
+  vid c;
+  AT a;
+};
+
 // Per-graph control block (device). One per graph of a batch; a plain run has one.
 struct GState {
   double mass;        // 2m as double (reference "mass", host_graph.cpp:112-114)

@@ -93,7 +93,7 @@ static size_t wsize(int mode) {  // bytes of one stored edge weight
 }
 static bool mode_exact(int mode) { return mode == MODE_UNIT || mode == MODE_INT; }
 static int coarse_mode(int mode) { return mode == MODE_UNIT ? MODE_INT : mode; }
-static int slot_bytes(int mode) {
+static int slot_bytes(int mode) {  // hash-slot value (partial sum)
   switch (mode) {
     case MODE_UNIT: return 4;
     case MODE_INT: return 8;
@@ -102,15 +102,22 @@ static int slot_bytes(int mode) {
   }
   return 8;
 }
+static int total_bytes(int mode, bool rec32) {  // community total carried in a record
+  return (mode == MODE_UNIT && rec32) ? 4 : 8;
+}
+// LDS bytes of one table slot: key + partial sum + community total
+static int slot_total_bytes(int mode, bool rec32) {
+  return 4 + slot_bytes(mode) + total_bytes(mode, rec32);
+}
 
 // Bin layout for a mode, with LDS tables shrunk to what the device allows.
-static BinLayout layout_for(const mgl_ctx* ctx, int mode) {
-  const int sb = slot_bytes(mode);
+static BinLayout layout_for(const mgl_ctx* ctx, int mode, bool rec32) {
+  const int sb = slot_total_bytes(mode, rec32);
   BinLayout L = ctx->have_user_bins ? ctx->user_bins : default_bins(ctx->lds_max, sb);
   for (int i = 0; i < L.nb; ++i) {
     BinSpec& b = L.b[i];
     if (b.shape == SHAPE_BLOCK) {
-      const int64_t cap = (int64_t)(ctx->lds_max - 2048) / (4 + sb);
+      const int64_t cap = (int64_t)(ctx->lds_max - 2048) / sb;
       if (b.size > cap) b.size = (int)cap;
       // every distinct community of a row needs a slot
       if (b.upper > b.size) b.upper = b.size;
@@ -170,7 +177,8 @@ struct Engine {
   int64_t n0 = 0, nnz0 = 0;
   std::vector<int64_t> vbase0;  // per-graph level-0 vertex ranges [G+1]
   // per-vertex state
-  DBuf bc, bp, bn, bk, ba, bvin, bperm, blab, bfin, bgid0;
+  DBuf bc, bp, bn, bk, ba, bvin, bperm, blab, bfin, bgid0, brec;
+  bool rec32 = false;  // 8-byte records (unit mode, every graph's 2m < 2^31)
   vid* c = nullptr; vid* p = nullptr; vid* nx = nullptr;
   // per-graph
   DBuf bgs, brun, btraj, bkeep, bdelta, bgather, bcounts;
@@ -179,7 +187,7 @@ struct Engine {
   std::vector<int64_t> bin_n, bin_off;
   // hub plan of the current level (last bin)
   DBuf hchunk, hv, hbk, hbkhub, htmpc, htmpw, hbktc, hbktw, hbcount, hboff, hbcur, heici, hself,
-      hbgain, hbdst, hgk, hgv, hscan;
+      hbgain, hbdst, hgk, hgv, hscan, htmpa, hbkta, hga;
   HubPlan hub;
   // coarsening buffers
   DBuf cflag, crank, cnewc, ck0, ck1, cv0, cv1, chist, chead, cuidx, crowcnt, cscan;
@@ -205,6 +213,7 @@ struct Engine {
       if (e) (void)hipEventDestroy(e);
   }
   hipStream_t st() const { return ctx->st; }
+  bool rec32_now() { return rec32 && LG().mode == MODE_UNIT; }
   // device word: graphs still active after the last decision
   int* nactive() { return (int*)(bcounts.as<unsigned long long>() + 3 * MGL_MAX_BINS); }
   LevelGraph& LG() { return lg[cur]; }
@@ -219,6 +228,7 @@ struct Engine {
     HIPC(ctx, ba.ensure(8 * n));
     HIPC(ctx, bvin.ensure(8 * n));
     HIPC(ctx, bperm.ensure(4 * n));
+    HIPC(ctx, brec.ensure(16 * n));
     HIPC(ctx, blab.ensure(4 * n));
     HIPC(ctx, bfin.ensure(4 * n));
     HIPC(ctx, bgs.ensure(sizeof(GState) * G));
@@ -272,7 +282,7 @@ struct Engine {
   // Binning of the current level (BIN::bin_create, BIN.cu:468-499).
   mgl_status make_bins() {
     LevelGraph& g = LG();
-    L = layout_for(ctx, g.mode);
+    L = layout_for(ctx, g.mode, rec32 && g.mode == MODE_UNIT);
     const int nb = L.nb;
     bin_n.assign(nb, 0);
     bin_off.assign(nb + 1, 0);
@@ -333,8 +343,12 @@ struct Engine {
       HIPC(ctx, hv.ensure(4 * m));
       HIPC(ctx, hbk.ensure(8 * (m + 1)));
       HIPC(ctx, hbkhub.ensure(8 * hub.nbuckets));
+      const size_t ab = (size_t)total_bytes(g.mode, rec32_now());
       HIPC(ctx, htmpc.ensure(4 * e));
       HIPC(ctx, hbktc.ensure(4 * e));
+      HIPC(ctx, htmpa.ensure(ab * e));
+      HIPC(ctx, hbkta.ensure(ab * e));
+      HIPC(ctx, hga.ensure(ab * 2 * e));
       if (g.mode != MODE_UNIT) {
         HIPC(ctx, htmpw.ensure(sb * e));
         HIPC(ctx, hbktw.ensure(sb * e));
@@ -359,6 +373,9 @@ struct Engine {
       hub.hub_bk = hbk.as<int64_t>();
       hub.bk_hub = hbkhub.as<int64_t>();
       hub.tmp_c = htmpc.as<vid>();
+      hub.tmp_a = htmpa.p;
+      hub.bkt_a = hbkta.p;
+      hub.ga = hga.p;
       hub.tmp_w = htmpw.p;
       hub.bkt_c = hbktc.as<vid>();
       hub.bkt_w = hbktw.p;
@@ -372,7 +389,8 @@ struct Engine {
       hub.gk = hgk.as<vid>();
       hub.gv = hgv.p;
       hub.scan_tmp = hscan.p;
-      hub.lds_slots = (uint32_t)((ctx->lds_max - 2048) / (4 + (int)sb) / 16 * 16);
+      hub.lds_slots =
+          (uint32_t)((ctx->lds_max - 2048) / slot_total_bytes(g.mode, rec32_now()) / 16 * 16);
     }
     return MGL_OK;
   }
@@ -386,6 +404,7 @@ struct Engine {
     A.k = bk.p;
     A.c = from;
     A.a = ba.p;
+    A.rec = brec.p;
     A.c_new = to;
     A.vin = vin;
     A.gid = g.gid.as<int32_t>();
@@ -402,6 +421,8 @@ struct Engine {
   mgl_status move(const vid* from, vid* to, int up, void* vin, bool time_bins) {
     LevelGraph& g = LG();
     MoveArgs A = move_args(from, to, up, vin);
+    const int r32 = rec32_now() ? 1 : 0;
+    launch_build_rec(g.mode, r32, g.n, from, ba.p, brec.p, st());
     int ne = 0;
     for (int b = L.nb - 1; b >= 1; --b) {
       if (bin_n[b] == 0) continue;
@@ -409,8 +430,8 @@ struct Engine {
       if (timing) A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS + b;
       if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b], st()));
       const int rc = (L.b[b].shape == SHAPE_GLOBAL)
-                         ? launch_hub(g.mode, A, hub, st())
-                         : launch_move_bin(g.mode, L.b[b], A, bperm.as<vid>() + bin_off[b],
+                         ? launch_hub(g.mode, r32, A, hub, st())
+                         : launch_move_bin(g.mode, r32, L.b[b], A, bperm.as<vid>() + bin_off[b],
                                            bin_n[b], st());
       if (rc != 0) FAIL(ctx, MGL_UNSUPPORTED, "no move kernel for bin %d (s=%d size=%d)", b, L.b[b].s, L.b[b].size);
       if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b + 1], st()));
@@ -1088,6 +1109,9 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
       s.mass = m;
     }
   }
+  // 8-byte records hold community totals in int32: every total is <= 2m
+  rec32 = (prec == MGL_PREC_EXACT);
+  for (int i = 0; i < G; ++i) rec32 = rec32 && hgs[i].mass_i < ((int64_t)1 << 31);
   // device union graph
   LevelGraph& g = lg[0];
   cur = 0;
@@ -1309,7 +1333,7 @@ void mgl_result_free(mgl_result* r) { delete r; }
 
 int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins) {
   if (!ctx) return 0;
-  BinLayout L = layout_for(ctx, MODE_UNIT);
+  BinLayout L = layout_for(ctx, MODE_UNIT, true);
   for (int i = 0; i < L.nb && i < max_bins; ++i) upper[i] = L.b[i].upper;
   return L.nb;
 }
@@ -1724,7 +1748,7 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
 // writes the fastest as a tuning file.
 static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
   std::vector<BinLayout> out;
-  const BinLayout D = default_bins(ctx->lds_max, 4);
+  const BinLayout D = default_bins(ctx->lds_max, slot_total_bytes(MODE_UNIT, true));
   out.push_back(D);
   // variants of the built-in layout: the shapes are kept, the degree bounds
   // between them move (two entries per lane, earlier workgroups, earlier hubs)
@@ -1759,7 +1783,7 @@ extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* p
   const BinLayout saved = ctx->user_bins;
   const bool had = ctx->have_user_bins;
   double best_ms = 1e300;
-  BinLayout best = default_bins(ctx->lds_max, 4);
+  BinLayout best = default_bins(ctx->lds_max, slot_total_bytes(MODE_UNIT, true));
   for (const BinLayout& cand : tuning_candidates(ctx)) {
     ctx->user_bins = cand;
     ctx->have_user_bins = true;

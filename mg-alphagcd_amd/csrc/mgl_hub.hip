@@ -42,14 +42,19 @@ __device__ __forceinline__ void hatom_add(int64_t* p, int64_t x) {
 __device__ __forceinline__ void hatom_add(float* p, float x) { atomicAdd(p, x); }
 __device__ __forceinline__ void hatom_add(double* p, double x) { atomicAdd(p, x); }
 
-// table of H slots (any H >= 1), linear probing
-template <class ST, bool GLOBAL>
-__device__ __forceinline__ void tab_insert(vid* K, ST* V, vid key, ST w, uint32_t H) {
+// table of H slots (any H >= 1), linear probing; the creator of a key stores
+// its community total
+template <class ST, class AT, bool GLOBAL>
+__device__ __forceinline__ void tab_insert(vid* K, ST* V, AT* Av, vid key, ST w, AT ak,
+                                           uint32_t H) {
   uint32_t h = hub_slot(key, H);
   while (true) {
     vid cur = GLOBAL ? __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                      : __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == MGL_EMPTY) cur = atomicCAS(&K[h], MGL_EMPTY, key);
+    if (cur == MGL_EMPTY) {
+      cur = atomicCAS(&K[h], MGL_EMPTY, key);
+      if (cur == MGL_EMPTY) Av[h] = ak;
+    }
     if (cur == MGL_EMPTY || cur == key) { hatom_add(&V[h], w); return; }
     h = (h + 1u == H) ? 0u : h + 1u;
   }
@@ -68,7 +73,7 @@ __device__ __forceinline__ T bsum256(T v, T* red) {
 }
 
 // ---- count ------------------------------------------------------------------
-template <class M>
+template <class M, class AT>
 __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
   typedef typename M::S ST;
   typedef typename M::K KT;
@@ -86,17 +91,20 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
   const uint32_t P = (uint32_t)(H.hub_bk[i + 1] - H.hub_bk[i]);
   for (uint32_t b = threadIdx.x; b < P; b += HUB_T) hist[b] = 0;
   __syncthreads();
-  const vid ci = A.c[v];
+  const Rec<AT>* R = (const Rec<AT>*)A.rec;
+  const vid ci = R[v].c;
   const bool v1 = A.version == 1;
   KT eici = 0, selfw = 0;
   for (int64_t e = e0 + threadIdx.x; e < e1; e += HUB_T) {
     const vid u = A.col[e];
-    const vid cj = A.c[u];
+    const Rec<AT> r = R[u];
+    const vid cj = r.c;
     const ST wv = wload<M>(A.w, e);
     const int64_t pos = pos0 + (e - e0);
     const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
     if (cand) {
       H.tmp_c[pos] = cj;
+      ((AT*)H.tmp_a)[pos] = r.a;
       if (M::HAS_W) ((ST*)H.tmp_w)[pos] = wv;
       atomicAdd(&hist[hub_bucket(cj, P)], 1u);
     } else {
@@ -119,7 +127,7 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
 }
 
 // ---- scatter ----------------------------------------------------------------
-template <class M>
+template <class M, class AT>
 __global__ __launch_bounds__(HUB_T) void k_hub_scatter(MoveArgs A, HubPlan H) {
   typedef typename M::S ST;
   __shared__ unsigned int cnt[HUB_MAXP];
@@ -157,13 +165,20 @@ __global__ __launch_bounds__(HUB_T) void k_hub_scatter(MoveArgs A, HubPlan H) {
   for (int j = 0; j < HUB_ITEMS; ++j) {
     if (key[j] == MGL_EMPTY) continue;
     const int64_t dst = base[bk[j]] + rk[j];
+    const int64_t src = pos0 + (int64_t)j * HUB_T + threadIdx.x;
     H.bkt_c[dst] = key[j];
-    if (M::HAS_W) ((ST*)H.bkt_w)[dst] = ((const ST*)H.tmp_w)[pos0 + (int64_t)j * HUB_T + threadIdx.x];
+    ((AT*)H.bkt_a)[dst] = ((const AT*)H.tmp_a)[src];
+    if (M::HAS_W) ((ST*)H.bkt_w)[dst] = ((const ST*)H.tmp_w)[src];
   }
 }
 
 // ---- aggregate ----------------------------------------------------------------
-template <class M>
+// LDS / HBM table layout: keys, values, totals in 16-byte aligned parts
+__host__ __device__ inline size_t hub_tab_bytes(uint32_t H, size_t sv, size_t sa) {
+  return ((4 * (size_t)H + 15) / 16) * 16 + ((sv * H + 15) / 16) * 16 + sa * H;
+}
+
+template <class M, class AT>
 __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hlds) {
   typedef typename M::S ST;
   typedef typename M::K KT;
@@ -196,38 +211,47 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
     const uint32_t Ht = lds ? Hlds : (uint32_t)(2 * cnt);
     vid* K;
     ST* V;
+    AT* Av;
     if (lds) {
       K = (vid*)smem;
       V = (ST*)(smem + (((size_t)4 * Hlds + 15) / 16) * 16);
+      Av = (AT*)((unsigned char*)V + ((sizeof(ST) * (size_t)Hlds + 15) / 16) * 16);
     } else {
       K = H.gk + 2 * s0;
       V = (ST*)H.gv + 2 * s0;
+      Av = (AT*)H.ga + 2 * s0;
     }
     for (uint32_t s = threadIdx.x; s < Ht; s += 1024) { K[s] = MGL_EMPTY; V[s] = (ST)0; }
     if (!lds) __threadfence();
     __syncthreads();
+    const AT* ba = (const AT*)H.bkt_a;
     for (int64_t j = s0 + threadIdx.x; j < s1; j += 1024) {
       const vid key = H.bkt_c[j];
       const ST wv = M::HAS_W ? ((const ST*)H.bkt_w)[j] : (ST)1;
-      if (lds) tab_insert<ST, false>(K, V, key, wv, Ht);
-      else tab_insert<ST, true>(K, V, key, wv, Ht);
+      if (lds) tab_insert<ST, AT, false>(K, V, Av, key, wv, ba[j], Ht);
+      else tab_insert<ST, AT, true>(K, V, Av, key, wv, ba[j], Ht);
     }
     if (!lds) __threadfence();
     __syncthreads();
     for (uint32_t s = threadIdx.x; s < Ht; s += 1024) {
       vid cj;
       ST ev;
+      AT ak = 0;
       if (lds) {
         cj = K[s];
         ev = V[s];
+        ak = Av[s];
       } else {
         cj = __hip_atomic_load(&K[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ev = (cj != MGL_EMPTY) ? __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : (ST)0;
+        if (cj != MGL_EMPTY) {
+          ev = __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ak = __hip_atomic_load(&Av[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
       }
       if (cj != MGL_EMPTY) {
         ++occ;
         double wij = (double)ev;
-        const double acj = (double)a[cj];
+        const double acj = (double)ak;
         wij -= (deici - dki * (daci - acj) / (2. * Mh));  // louvain.cu:540-543
         wij /= Mh;
         if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
@@ -253,7 +277,7 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
 }
 
 // ---- finalize -----------------------------------------------------------------
-template <class M>
+template <class M, class AT>
 __global__ void k_hub_final(MoveArgs A, HubPlan H) {
   typedef typename M::K KT;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -277,46 +301,46 @@ __global__ void k_hub_copy(const int64_t* s, int64_t* d, int64_t n) {
     d[j] = s[j];
 }
 
-template <class M>
+template <class M, class AT>
 static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t st) {
   typedef typename M::S ST;
   if (H.m <= 0) return 0;
   launch_fill_bytes0(H.bcount, 8 * (size_t)H.nbuckets, st);
   launch_fill_bytes0(H.hub_eici, 8 * (size_t)H.m, st);
   launch_fill_bytes0(H.hub_self, 8 * (size_t)H.m, st);
-  k_hub_count<M><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+  k_hub_count<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
   scan_exclusive_i64(H.bcount, H.boff, H.nbuckets, H.scan_tmp, st);
   {
     int64_t gr = (H.nbuckets + 255) / 256;
     if (gr > 4096) gr = 4096;
     k_hub_copy<<<(unsigned)gr, 256, 0, st>>>(H.boff, H.bcur, H.nbuckets);
   }
-  k_hub_scatter<M><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
-  const size_t lds = (((size_t)4 * Hlds + 15) / 16) * 16 + sizeof(ST) * (size_t)Hlds;
-  k_hub_aggregate<M><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
-  k_hub_final<M><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
+  k_hub_scatter<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+  const size_t lds = hub_tab_bytes(Hlds, sizeof(ST), sizeof(AT));
+  k_hub_aggregate<M, AT><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+  k_hub_final<M, AT><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
   return 0;
 }
 
-int launch_hub(int mode, const MoveArgs& A, const HubPlan& H, hipStream_t st) {
+int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStream_t st) {
   switch (mode) {
-    case MODE_UNIT: return hub_t<ModeUnit>(A, H, H.lds_slots, st);
-    case MODE_INT: return hub_t<ModeInt>(A, H, H.lds_slots, st);
-    case MODE_F64: return hub_t<ModeF64>(A, H, H.lds_slots, st);
-    case MODE_F32: return hub_t<ModeF32>(A, H, H.lds_slots, st);
+    case MODE_UNIT:
+      return rec32 ? hub_t<ModeUnit, int32_t>(A, H, H.lds_slots, st)
+                   : hub_t<ModeUnit, int64_t>(A, H, H.lds_slots, st);
+    case MODE_INT: return hub_t<ModeInt, int64_t>(A, H, H.lds_slots, st);
+    case MODE_F64: return hub_t<ModeF64, double>(A, H, H.lds_slots, st);
+    case MODE_F32: return hub_t<ModeF32, double>(A, H, H.lds_slots, st);
   }
   return -1;
 }
 
 void hub_allow_lds(int bytes) {
-  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<ModeUnit>,
+#define A_(M_, AT_)                                                        \
+  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_>,         \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<ModeInt>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<ModeF64>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<ModeF32>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  A_(ModeUnit, int32_t) A_(ModeUnit, int64_t) A_(ModeInt, int64_t) A_(ModeF64, double)
+  A_(ModeF32, double)
+#undef A_
 }
 
 int hub_chunk_edges() { return HUB_CHUNK; }

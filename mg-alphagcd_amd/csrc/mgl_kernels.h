@@ -35,6 +35,7 @@ struct MoveArgs {
   const void* k;      // K[n]
   const vid* c;       // assignment the moves start from
   const void* a;      // K[n] community totals of c
+  const void* rec;    // Rec<AT>[n]: {c[u], a[c[u]]} of the assignment the moves start from
   vid* c_new;         // proposal
   void* vin;          // K[n]: per-vertex weight to own community incl. self-loop (nullable)
   const int32_t* gid; // per-vertex graph id (nullable = graph 0)
@@ -47,8 +48,11 @@ struct MoveArgs {
 
 // Launch the move kernel of one GROUP or BLOCK bin. perm points at the bin's
 // vertices. Returns 0, or -1 when no kernel instance matches the bin's shape.
-int launch_move_bin(int mode, const BinSpec& bs, const MoveArgs& A, const vid* perm,
+int launch_move_bin(int mode, int rec32, const BinSpec& bs, const MoveArgs& A, const vid* perm,
                     int64_t nb, hipStream_t st);
+// rec[v] = {c[v], a[c[v]]}; rec32 selects 8-byte records (unit mode, 2m < 2^31)
+void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
+                      hipStream_t st);
 void move_kernels_allow_lds(int bytes);
 
 // ---- hub path (mgl_hub.hip) ------------------------------------------------
@@ -59,6 +63,8 @@ struct HubPlan {
   const int64_t* hub_bk = nullptr;  // [m+1] bucket ranges
   const int64_t* bk_hub = nullptr;  // [nbuckets] owning hub
   vid* tmp_c = nullptr;             // [ehub] neighbour community or EMPTY
+  void* tmp_a = nullptr;            // [ehub] its community total
+  void* bkt_a = nullptr;            // [ehub] bucketed totals
   void* tmp_w = nullptr;            // [ehub] weights (weighted modes)
   vid* bkt_c = nullptr;             // [ehub] bucketed candidates
   void* bkt_w = nullptr;
@@ -71,10 +77,11 @@ struct HubPlan {
   vid* bdst = nullptr;              // [nbuckets]
   vid* gk = nullptr;                // [2*ehub] HBM tables for oversized buckets
   void* gv = nullptr;
+  void* ga = nullptr;
   void* scan_tmp = nullptr;
   uint32_t lds_slots = 0;           // LDS table slots of the aggregate kernel
 };
-int launch_hub(int mode, const MoveArgs& A, const HubPlan& H, hipStream_t st);
+int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStream_t st);
 void hub_allow_lds(int bytes);
 int hub_chunk_edges();
 int hub_max_buckets();

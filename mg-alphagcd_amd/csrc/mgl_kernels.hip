@@ -19,6 +19,13 @@
 #include "mgl_kernels.h"
 #include <climits>
 
+static unsigned grid_for(int64_t n, int per_block, unsigned cap) {
+  int64_t g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
 // ---------------------------------------------------------------------------
 // atomics on any slot type
 __device__ __forceinline__ void atom_add(int32_t* p, int32_t x) { atomicAdd(p, x); }
@@ -27,34 +34,6 @@ __device__ __forceinline__ void atom_add(int64_t* p, int64_t x) {
 }
 __device__ __forceinline__ void atom_add(float* p, float x) { atomicAdd(p, x); }
 __device__ __forceinline__ void atom_add(double* p, double x) { atomicAdd(p, x); }
-
-// Insert (key, w) into an LDS open-addressing table of 2^lg slots. Keys are
-// written once (EMPTY -> key) and never change afterwards, so a plain read that
-// sees a key is final; a stale EMPTY only costs a CAS that returns the key.
-template <class ST>
-__device__ __forceinline__ void lds_insert(vid* K, ST* V, vid key, ST w, int lg) {
-  const uint32_t mask = (1u << lg) - 1u;
-  uint32_t h = slot_hash(key, lg);
-  while (true) {
-    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == MGL_EMPTY) cur = atomicCAS(&K[h], MGL_EMPTY, key);
-    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
-    h = (h + 1u) & mask;
-  }
-}
-
-// Same for a table in HBM; reads are agent-coherent loads (no stale L1 lines).
-template <class ST>
-__device__ __forceinline__ void glb_insert(vid* K, ST* V, vid key, ST w, int lg) {
-  const uint32_t mask = (1u << lg) - 1u;
-  uint32_t h = slot_hash(key, lg);
-  while (true) {
-    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (cur == MGL_EMPTY) cur = atomicCAS(&K[h], MGL_EMPTY, key);
-    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
-    h = (h + 1u) & mask;
-  }
-}
 
 template <class M>
 __device__ __forceinline__ bool vertex_active(const MoveArgs& A, vid v, int& g) {
@@ -78,9 +57,111 @@ __device__ __forceinline__ vid clamp_dst(vid dst, vid ci, int up, int version) {
 }
 
 // ---------------------------------------------------------------------------
+// Per-vertex record {c[u], a[c[u]]}, rebuilt before every move pass, so that
+// one gather per CSR entry brings both the neighbour's community and that
+// community's total. The slot creator stores the total in the table, so the
+// gain loop reads no global memory. AT = int32 in the unit mode while
+// 2m < 2^31 (an 8-byte record), else the mode's total type (16 bytes).
+template <class KT, class AT>
+__global__ __launch_bounds__(256) void k_build_rec(int64_t n, const vid* __restrict__ c,
+                                                   const KT* __restrict__ a,
+                                                   Rec<AT>* __restrict__ rec) {
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * 256) {
+    const vid x = c[v];
+    Rec<AT> r;
+    r.c = x;
+    r.a = (AT)a[x];
+    rec[v] = r;
+  }
+}
+
+void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
+                      hipStream_t st) {
+  if (n <= 0) return;
+  const unsigned grid = grid_for(n, 256, 8192);
+  if (mode == MODE_UNIT && rec32)
+    k_build_rec<int64_t, int32_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a, (Rec<int32_t>*)rec);
+  else if (mode == MODE_UNIT || mode == MODE_INT)
+    k_build_rec<int64_t, int64_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a, (Rec<int64_t>*)rec);
+  else
+    k_build_rec<double, double><<<grid, 256, 0, st>>>(n, c, (const double*)a, (Rec<double>*)rec);
+}
+
+// Insert (key, w) into a table of 2^lg slots; the creator of a key also
+// stores its community total. Keys are written once (EMPTY -> key).
+template <class ST, class AT>
+__device__ __forceinline__ void ins_pow2(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, int lg) {
+  const uint32_t mask = (1u << lg) - 1u;
+  uint32_t h = slot_hash(key, lg);
+  while (true) {
+    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == MGL_EMPTY) {
+      cur = atomicCAS(&K[h], MGL_EMPTY, key);
+      if (cur == MGL_EMPTY) Av[h] = ak;
+    }
+    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
+    h = (h + 1u) & mask;
+  }
+}
+// Same for a table of any size H (multiply-high slot).
+template <class ST, class AT>
+__device__ __forceinline__ void ins_any(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, uint32_t H) {
+  uint32_t h = any_slot(key, H);
+  while (true) {
+    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (cur == MGL_EMPTY) {
+      cur = atomicCAS(&K[h], MGL_EMPTY, key);
+      if (cur == MGL_EMPTY) Av[h] = ak;
+    }
+    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
+    h = (h + 1u == H) ? 0u : h + 1u;
+  }
+}
+
+#define MOVE_UNROLL 4
+
+// Walks the row of v with `stride` lanes starting at `lane`, MOVE_UNROLL
+// entries per lane per step (their gathers are issued together), inserting
+// candidates and summing the own community's weight (eici, self-loop).
+template <class M, class AT, bool POW2>
+__device__ __forceinline__ void scan_row(const MoveArgs& A, vid v, vid ci, eid e0, eid e1,
+                                         int lane, int stride, vid* K, typename M::S* V, AT* Av,
+                                         uint32_t tab, typename M::S& eici,
+                                         typename M::S& selfw) {
+  typedef typename M::S ST;
+  const Rec<AT>* R = (const Rec<AT>*)A.rec;
+  const bool v1 = A.version == 1;
+  for (eid e = e0 + lane; e < e1; e += (eid)stride * MOVE_UNROLL) {
+    vid u[MOVE_UNROLL];
+    Rec<AT> r[MOVE_UNROLL];
+#pragma unroll
+    for (int j = 0; j < MOVE_UNROLL; ++j) {
+      const eid ej = e + (eid)j * stride;
+      u[j] = ej < e1 ? A.col[ej] : MGL_EMPTY;
+    }
+#pragma unroll
+    for (int j = 0; j < MOVE_UNROLL; ++j)
+      if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
+#pragma unroll
+    for (int j = 0; j < MOVE_UNROLL; ++j) {
+      if (u[j] == MGL_EMPTY) continue;
+      const ST wv = wload<M>(A.w, e + (eid)j * stride);
+      const vid cj = r[j].c;
+      const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
+      if (cand) {
+        if (POW2) ins_pow2<ST, AT>(K, V, Av, cj, wv, r[j].a, (int)tab);
+        else ins_any<ST, AT>(K, V, Av, cj, wv, r[j].a, tab);
+      } else if (cj == ci) {
+        if (u[j] != v) eici += wv; else selfw += wv;
+      }
+    }
+  }
+}
+
 // SHAPE_GROUP: S lanes per vertex, 256/S vertices per workgroup, 2^LG LDS
 // slots per vertex.
-template <class M, int S, int LG>
+template <class M, class AT, int S, int LG>
 __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __restrict__ perm,
                                                     int64_t nb) {
   typedef typename M::S ST;
@@ -89,11 +170,13 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
   constexpr int H = 1 << LG;
   __shared__ vid sk[GPB * H];
   __shared__ ST sv[GPB * H];
+  __shared__ AT sa[GPB * H];
   __shared__ unsigned long long sdist;
   const int grp = threadIdx.x / S, lane = threadIdx.x & (S - 1);
   const int64_t gi = (int64_t)blockIdx.x * GPB + grp;
   vid* K = sk + grp * H;
   ST* V = sv + grp * H;
+  AT* Av = sa + grp * H;
 #pragma unroll
   for (int i = lane; i < H; i += S) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
   if (threadIdx.x == 0) sdist = 0;
@@ -104,18 +187,12 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
   __syncthreads();
   ST eici = 0, selfw = 0;
   vid ci = 0;
+  KT acik = 0;
   if (ok) {
-    ci = A.c[v];
-    const eid e0 = A.off[v], e1 = A.off[v + 1];
-    const bool v1 = A.version == 1;
-    for (eid e = e0 + lane; e < e1; e += S) {
-      const vid u = A.col[e];
-      const vid cj = A.c[u];
-      const ST wv = wload<M>(A.w, e);
-      const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
-      if (cand) lds_insert<ST>(K, V, cj, wv, LG);
-      else if (cj == ci) { if (u != v) eici += wv; else selfw += wv; }
-    }
+    const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
+    ci = rv.c;
+    acik = (KT)rv.a;
+    scan_row<M, AT, true>(A, v, ci, A.off[v], A.off[v + 1], lane, S, K, V, Av, LG, eici, selfw);
   }
   __syncthreads();
   if (ok) {
@@ -123,8 +200,7 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
     selfw = group_sum<S>(selfw);
     const double Mh = A.gs[g].mass / 2.0;
     const KT ki = ((const KT*)A.k)[v];
-    const KT* a = (const KT*)A.a;
-    const KT aci_k = a[ci] - ki;
+    const KT aci_k = acik - ki;
     const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
     const bool small = A.tie_small ? true : (A.up != 0);
     double best = 0.0;
@@ -135,7 +211,7 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
       const vid cj = K[i];
       if (cj != MGL_EMPTY) {
         ++cnt;
-        const double wij = gain((double)V[i], deici, dki, daci, (double)a[cj], Mh);
+        const double wij = gain((double)V[i], deici, dki, daci, (double)Av[i], Mh);
         if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
       }
     }
@@ -183,19 +259,13 @@ __device__ __forceinline__ void block_argmax(double& bg, vid& bd, bool small, do
     if (better(rg[i], rd[i], bg, bd, small)) { bg = rg[i]; bd = rd[i]; }
 }
 
-// SHAPE_BLOCK: one workgroup per vertex, an LDS table of H slots (any H).
-template <class ST>
-__device__ __forceinline__ void lds_insert_any(vid* K, ST* V, vid key, ST w, uint32_t H) {
-  uint32_t h = any_slot(key, H);
-  while (true) {
-    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == MGL_EMPTY) cur = atomicCAS(&K[h], MGL_EMPTY, key);
-    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
-    h = (h + 1u == H) ? 0u : h + 1u;
-  }
+// LDS layout of a table of H slots: keys, values, totals (16-byte aligned parts)
+__host__ __device__ inline size_t tab_bytes(uint32_t H, size_t sv, size_t sa) {
+  return ((4 * (size_t)H + 15) / 16) * 16 + ((sv * H + 15) / 16) * 16 + sa * H;
 }
 
-template <class M, int BS>
+// SHAPE_BLOCK: one workgroup per vertex, an LDS table of H slots (any H).
+template <class M, class AT, int BS>
 __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __restrict__ perm,
                                                    int64_t nb, uint32_t H) {
   typedef typename M::S ST;
@@ -211,30 +281,21 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   int g;
   if (!vertex_active<M>(A, v, g)) return;  // uniform over the workgroup
   vid* K = (vid*)smem;
-  ST* V = (ST*)(smem + (((size_t)4 * H + 15) / 16) * 16);
+  ST* V = (ST*)(smem + ((4 * (size_t)H + 15) / 16) * 16);
+  AT* Av = (AT*)((unsigned char*)V + ((sizeof(ST) * (size_t)H + 15) / 16) * 16);
   for (uint32_t i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
+  const vid ci = rv.c;
   __syncthreads();
-  const vid ci = A.c[v];
-  const eid e0 = A.off[v], e1 = A.off[v + 1];
-  const bool v1 = A.version == 1;
   ST eici = 0, selfw = 0;
-  for (eid e = e0 + threadIdx.x; e < e1; e += BS) {
-    const vid u = A.col[e];
-    const vid cj = A.c[u];
-    const ST wv = wload<M>(A.w, e);
-    const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
-    if (cand) lds_insert_any<ST>(K, V, cj, wv, H);
-    else if (cj == ci) {
-      if (u != v) eici += wv; else selfw += wv;
-    }
-  }
+  scan_row<M, AT, false>(A, v, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, K, V, Av, H, eici,
+                         selfw);
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
   selfw = block_sum<BS>(selfw, rs);
   const double Mh = A.gs[g].mass / 2.0;
   const KT ki = ((const KT*)A.k)[v];
-  const KT* a = (const KT*)A.a;
-  const KT aci_k = a[ci] - ki;
+  const KT aci_k = (KT)rv.a - ki;
   const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
   const bool small = A.tie_small ? true : (A.up != 0);
   double best = 0.0;
@@ -244,7 +305,7 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
     const vid cj = K[i];
     if (cj != MGL_EMPTY) {
       ++cnt;
-      const double wij = gain((double)V[i], deici, dki, daci, (double)a[cj], Mh);
+      const double wij = gain((double)V[i], deici, dki, daci, (double)Av[i], Mh);
       if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
     }
   }
@@ -272,14 +333,14 @@ BinLayout default_bins(int lds_bytes, int slot_bytes) {
   L.b[i++] = {1024, SHAPE_BLOCK, 256, 2048};
   L.b[i++] = {4096, SHAPE_BLOCK, 1024, 8192};
   // the largest table the workgroup LDS holds, filled to at most 3/4
-  const int64_t hmax = (int64_t)(lds_bytes - 2048) / (4 + slot_bytes) / 16 * 16;
+  const int64_t hmax = (int64_t)(lds_bytes - 2048) / slot_bytes / 16 * 16;
   if (hmax * 3 / 4 > 4096) L.b[i++] = {hmax * 3 / 4, SHAPE_BLOCK, 1024, (int)hmax};
   L.b[i++] = {INT64_MAX, SHAPE_GLOBAL, 0, 0};
   L.nb = i;
   return L;
 }
 
-template <class M>
+template <class M, class AT>
 static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int64_t nb,
                       hipStream_t st) {
   typedef typename M::S ST;
@@ -289,7 +350,7 @@ static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int
     const unsigned grid = (unsigned)((nb + gpb - 1) / gpb);
 #define G_(S_, LG_)                                                                     \
   if (bs.s == S_ && bs.size == LG_) {                                                   \
-    k_move_group<M, S_, LG_><<<grid, 256, 0, st>>>(A, perm, nb);                        \
+    k_move_group<M, AT, S_, LG_><<<grid, 256, 0, st>>>(A, perm, nb);                    \
     return 0;                                                                           \
   }
     G_(4, 3) G_(4, 4) G_(8, 4) G_(8, 5) G_(16, 5) G_(16, 6) G_(32, 6) G_(32, 7)
@@ -297,43 +358,45 @@ static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int
 #undef G_
   } else if (bs.shape == SHAPE_BLOCK) {
     const uint32_t H = (uint32_t)bs.size;
-    const size_t lds = (((size_t)4 * H + 15) / 16) * 16 + sizeof(ST) * (size_t)H;
+    const size_t lds = tab_bytes(H, sizeof(ST), sizeof(AT));
     if (bs.s == 256) {
-      k_move_block<M, 256><<<(unsigned)nb, 256, lds, st>>>(A, perm, nb, H);
+      k_move_block<M, AT, 256><<<(unsigned)nb, 256, lds, st>>>(A, perm, nb, H);
       return 0;
     }
     if (bs.s == 512) {
-      k_move_block<M, 512><<<(unsigned)nb, 512, lds, st>>>(A, perm, nb, H);
+      k_move_block<M, AT, 512><<<(unsigned)nb, 512, lds, st>>>(A, perm, nb, H);
       return 0;
     }
     if (bs.s == 1024) {
-      k_move_block<M, 1024><<<(unsigned)nb, 1024, lds, st>>>(A, perm, nb, H);
+      k_move_block<M, AT, 1024><<<(unsigned)nb, 1024, lds, st>>>(A, perm, nb, H);
       return 0;
     }
   }
   return -1;
 }
 
-int launch_move_bin(int mode, const BinSpec& bs, const MoveArgs& A, const vid* perm,
+int launch_move_bin(int mode, int rec32, const BinSpec& bs, const MoveArgs& A, const vid* perm,
                     int64_t nb, hipStream_t st) {
   switch (mode) {
-    case MODE_UNIT: return move_bin_t<ModeUnit>(bs, A, perm, nb, st);
-    case MODE_INT: return move_bin_t<ModeInt>(bs, A, perm, nb, st);
-    case MODE_F64: return move_bin_t<ModeF64>(bs, A, perm, nb, st);
-    case MODE_F32: return move_bin_t<ModeF32>(bs, A, perm, nb, st);
+    case MODE_UNIT:
+      return rec32 ? move_bin_t<ModeUnit, int32_t>(bs, A, perm, nb, st)
+                   : move_bin_t<ModeUnit, int64_t>(bs, A, perm, nb, st);
+    case MODE_INT: return move_bin_t<ModeInt, int64_t>(bs, A, perm, nb, st);
+    case MODE_F64: return move_bin_t<ModeF64, double>(bs, A, perm, nb, st);
+    case MODE_F32: return move_bin_t<ModeF32, double>(bs, A, perm, nb, st);
   }
   return -1;
 }
 
 // Opt the LDS-table kernels into the device's largest workgroup LDS size.
 void move_kernels_allow_lds(int bytes) {
-#define A_(M_, BS_) \
-  (void)hipFuncSetAttribute((const void*)k_move_block<M_, BS_>, \
+#define A_(M_, AT_, BS_) \
+  (void)hipFuncSetAttribute((const void*)k_move_block<M_, AT_, BS_>, \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  A_(ModeUnit, 256) A_(ModeUnit, 512) A_(ModeUnit, 1024)
-  A_(ModeInt, 256) A_(ModeInt, 512) A_(ModeInt, 1024)
-  A_(ModeF64, 256) A_(ModeF64, 512) A_(ModeF64, 1024)
-  A_(ModeF32, 256) A_(ModeF32, 512) A_(ModeF32, 1024)
+#define B_(M_, AT_) A_(M_, AT_, 256) A_(M_, AT_, 512) A_(M_, AT_, 1024)
+  B_(ModeUnit, int32_t) B_(ModeUnit, int64_t) B_(ModeInt, int64_t) B_(ModeF64, double)
+  B_(ModeF32, double)
+#undef B_
 #undef A_
 }
 
@@ -476,12 +539,6 @@ __global__ __launch_bounds__(256) void k_bin_scatter(const eid* __restrict__ off
   }
 }
 
-static unsigned grid_for(int64_t n, int per_block, unsigned cap) {
-  int64_t g = (n + per_block - 1) / per_block;
-  if (g < 1) g = 1;
-  if (g > cap) g = cap;
-  return (unsigned)g;
-}
 
 void launch_bin_count(const eid* off, int64_t n, const BinLayout& L, unsigned long long* counts,
                       hipStream_t st) {
