@@ -111,13 +111,17 @@ static int slot_total_bytes(int mode, bool rec32) {
 }
 
 // Bin layout for a mode, with LDS tables shrunk to what the device allows.
+// BLOCK bins of at most 16 entries per thread use the owner-based kernel
+// (key + partial sum per slot); longer ones also keep the total in the slot.
 static BinLayout layout_for(const mgl_ctx* ctx, int mode, bool rec32) {
-  const int sb = slot_total_bytes(mode, rec32);
+  const int sb = 4 + slot_bytes(mode);
   BinLayout L = ctx->have_user_bins ? ctx->user_bins : default_bins(ctx->lds_max, sb);
   for (int i = 0; i < L.nb; ++i) {
     BinSpec& b = L.b[i];
     if (b.shape == SHAPE_BLOCK) {
-      const int64_t cap = (int64_t)(ctx->lds_max - 2048) / sb;
+      const bool owner = (b.upper + b.s - 1) / b.s <= 16;
+      const int64_t cap =
+          (int64_t)(ctx->lds_max - 2048) / (owner ? sb : slot_total_bytes(mode, rec32));
       if (b.size > cap) b.size = (int)cap;
       // every distinct community of a row needs a slot
       if (b.upper > b.size) b.upper = b.size;
@@ -1748,7 +1752,7 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
 // writes the fastest as a tuning file.
 static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
   std::vector<BinLayout> out;
-  const BinLayout D = default_bins(ctx->lds_max, slot_total_bytes(MODE_UNIT, true));
+  const BinLayout D = default_bins(ctx->lds_max, 4 + slot_bytes(MODE_UNIT));
   out.push_back(D);
   // variants of the built-in layout: the shapes are kept, the degree bounds
   // between them move (two entries per lane, earlier workgroups, earlier hubs)
@@ -1783,7 +1787,7 @@ extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* p
   const BinLayout saved = ctx->user_bins;
   const bool had = ctx->have_user_bins;
   double best_ms = 1e300;
-  BinLayout best = default_bins(ctx->lds_max, slot_total_bytes(MODE_UNIT, true));
+  BinLayout best = default_bins(ctx->lds_max, 4 + slot_bytes(MODE_UNIT));
   for (const BinLayout& cand : tuning_candidates(ctx)) {
     ctx->user_bins = cand;
     ctx->have_user_bins = true;

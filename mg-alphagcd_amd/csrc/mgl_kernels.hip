@@ -319,6 +319,205 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Owner-based variants (rows of at most IT entries per lane): each lane keeps
+// in registers the entries it gathered and the slots whose key it created.
+// The table holds key and partial sum only; after the barrier every creator
+// evaluates its own keys with the total it gathered itself, so neither a
+// table scan nor a second gather is needed.
+template <class ST>
+__device__ __forceinline__ int own_pow2(vid* K, ST* V, vid key, ST w, int lg) {
+  const uint32_t mask = (1u << lg) - 1u;
+  uint32_t h = slot_hash(key, lg);
+  while (true) {
+    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    bool mine = false;
+    if (cur == MGL_EMPTY) {
+      cur = atomicCAS(&K[h], MGL_EMPTY, key);
+      mine = (cur == MGL_EMPTY);
+    }
+    if (mine || cur == key) { atom_add(&V[h], w); return mine ? (int)h : -1; }
+    h = (h + 1u) & mask;
+  }
+}
+template <class ST>
+__device__ __forceinline__ int own_any(vid* K, ST* V, vid key, ST w, uint32_t H) {
+  uint32_t h = any_slot(key, H);
+  while (true) {
+    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    bool mine = false;
+    if (cur == MGL_EMPTY) {
+      cur = atomicCAS(&K[h], MGL_EMPTY, key);
+      mine = (cur == MGL_EMPTY);
+    }
+    if (mine || cur == key) { atom_add(&V[h], w); return mine ? (int)h : -1; }
+    h = (h + 1u == H) ? 0u : h + 1u;
+  }
+}
+
+// Gathers and inserts the (at most IT per lane) entries of v's row.
+template <class M, class AT, int IT, bool POW2>
+__device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0, eid e1,
+                                        int lane, int stride, vid* K, typename M::S* V,
+                                        uint32_t tab, int (&own)[IT], AT (&oa)[IT],
+                                        typename M::S& eici, typename M::S& selfw) {
+  typedef typename M::S ST;
+  const Rec<AT>* R = (const Rec<AT>*)A.rec;
+  const bool v1 = A.version == 1;
+  vid u[IT];
+  Rec<AT> r[IT];
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    const eid e = e0 + lane + (eid)j * stride;
+    u[j] = e < e1 ? A.col[e] : MGL_EMPTY;
+  }
+#pragma unroll
+  for (int j = 0; j < IT; ++j)
+    if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    own[j] = -1;
+    if (u[j] == MGL_EMPTY) continue;
+    const eid e = e0 + lane + (eid)j * stride;
+    const ST wv = wload<M>(A.w, e);
+    const vid cj = r[j].c;
+    const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
+    if (cand) {
+      own[j] = POW2 ? own_pow2<ST>(K, V, cj, wv, (int)tab) : own_any<ST>(K, V, cj, wv, tab);
+      oa[j] = r[j].a;
+    } else if (cj == ci) {
+      if (u[j] != v) eici += wv; else selfw += wv;
+    }
+  }
+}
+
+template <class M, class AT, int IT>
+__device__ __forceinline__ void own_gains(const vid* K, const typename M::S* V,
+                                          const int (&own)[IT], const AT (&oa)[IT],
+                                          double deici, double dki, double daci, double Mh,
+                                          bool small, double& best, vid& dst, int& cnt) {
+#pragma unroll
+  for (int j = 0; j < IT; ++j) {
+    if (own[j] < 0) continue;
+    ++cnt;
+    const vid cj = K[own[j]];
+    const double wij = gain((double)V[own[j]], deici, dki, daci, (double)oa[j], Mh);
+    if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+  }
+}
+
+template <class M, class AT, int S, int LG, int IT>
+__global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* __restrict__ perm,
+                                                        int64_t nb) {
+  typedef typename M::S ST;
+  typedef typename M::K KT;
+  constexpr int GPB = 256 / S;
+  constexpr int H = 1 << LG;
+  __shared__ vid sk[GPB * H];
+  __shared__ ST sv[GPB * H];
+  __shared__ unsigned long long sdist;
+  const int grp = threadIdx.x / S, lane = threadIdx.x & (S - 1);
+  const int64_t gi = (int64_t)blockIdx.x * GPB + grp;
+  vid* K = sk + grp * H;
+  ST* V = sv + grp * H;
+#pragma unroll
+  for (int i = lane; i < H; i += S) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  if (threadIdx.x == 0) sdist = 0;
+  vid v = 0;
+  int g = 0;
+  bool ok = gi < nb;
+  if (ok) { v = perm[gi]; ok = vertex_active<M>(A, v, g); }
+  __syncthreads();
+  ST eici = 0, selfw = 0;
+  vid ci = 0;
+  KT acik = 0;
+  int own[IT];
+  AT oa[IT];
+#pragma unroll
+  for (int j = 0; j < IT; ++j) own[j] = -1;
+  if (ok) {
+    const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
+    ci = rv.c;
+    acik = (KT)rv.a;
+    own_row<M, AT, IT, true>(A, v, ci, A.off[v], A.off[v + 1], lane, S, K, V, LG, own, oa,
+                             eici, selfw);
+  }
+  __syncthreads();
+  if (ok) {
+    eici = group_sum<S>(eici);
+    selfw = group_sum<S>(selfw);
+    const double Mh = A.gs[g].mass / 2.0;
+    const KT ki = ((const KT*)A.k)[v];
+    const KT aci_k = acik - ki;
+    const bool small = A.tie_small ? true : (A.up != 0);
+    double best = 0.0;
+    vid dst = ci;
+    int cnt = 0;
+    own_gains<M, AT, IT>(K, V, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small, best,
+                         dst, cnt);
+    group_argmax<S>(best, dst, small);
+    if (A.stats) {
+      cnt = group_sum<S>(cnt);
+      if (lane == 0) atomicAdd(&sdist, (unsigned long long)cnt);
+    }
+    if (lane == 0) {
+      A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
+      if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
+    }
+  }
+  if (A.stats) {
+    __syncthreads();
+    if (threadIdx.x == 0 && sdist) atomicAdd(A.stats, sdist);
+  }
+}
+
+template <class M, class AT, int BS, int IT>
+__global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __restrict__ perm,
+                                                       int64_t nb, uint32_t H) {
+  typedef typename M::S ST;
+  typedef typename M::K KT;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double rg[BS / 64];
+  __shared__ vid rd[BS / 64];
+  __shared__ ST rs[BS / 64];
+  __shared__ int rc[BS / 64];
+  const int64_t gi = blockIdx.x;
+  if (gi >= nb) return;
+  const vid v = perm[gi];
+  int g;
+  if (!vertex_active<M>(A, v, g)) return;  // uniform over the workgroup
+  vid* K = (vid*)smem;
+  ST* V = (ST*)(smem + ((4 * (size_t)H + 15) / 16) * 16);
+  for (uint32_t i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
+  const vid ci = rv.c;
+  __syncthreads();
+  ST eici = 0, selfw = 0;
+  int own[IT];
+  AT oa[IT];
+  own_row<M, AT, IT, false>(A, v, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, K, V, H, own, oa,
+                            eici, selfw);
+  __syncthreads();
+  eici = block_sum<BS>(eici, rs);
+  selfw = block_sum<BS>(selfw, rs);
+  const double Mh = A.gs[g].mass / 2.0;
+  const KT ki = ((const KT*)A.k)[v];
+  const KT aci_k = (KT)rv.a - ki;
+  const bool small = A.tie_small ? true : (A.up != 0);
+  double best = 0.0;
+  vid dst = ci;
+  int cnt = 0;
+  own_gains<M, AT, IT>(K, V, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small, best,
+                       dst, cnt);
+  block_argmax<BS>(best, dst, small, rg, rd);
+  if (A.stats) cnt = block_sum<BS>(cnt, rc);
+  if (threadIdx.x == 0) {
+    A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
+    if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
+    if (A.stats) atomicAdd(A.stats, (unsigned long long)cnt);
+  }
+}
+
+// ---------------------------------------------------------------------------
 BinLayout default_bins(int lds_bytes, int slot_bytes) {
   BinLayout L;
   int i = 0;
@@ -332,9 +531,12 @@ BinLayout default_bins(int lds_bytes, int slot_bytes) {
   L.b[i++] = {256, SHAPE_GROUP, 64, 9};
   L.b[i++] = {1024, SHAPE_BLOCK, 256, 2048};
   L.b[i++] = {4096, SHAPE_BLOCK, 1024, 8192};
-  // the largest table the workgroup LDS holds, filled to at most 3/4
+  // the largest table the workgroup LDS holds, filled to at most 3/4, and at
+  // most 16 entries per thread (owner-based kernel)
   const int64_t hmax = (int64_t)(lds_bytes - 2048) / slot_bytes / 16 * 16;
-  if (hmax * 3 / 4 > 4096) L.b[i++] = {hmax * 3 / 4, SHAPE_BLOCK, 1024, (int)hmax};
+  int64_t up = hmax * 3 / 4;
+  if (up > 16 * 1024) up = 16 * 1024;
+  if (up > 4096) L.b[i++] = {up, SHAPE_BLOCK, 1024, (int)hmax};
   L.b[i++] = {INT64_MAX, SHAPE_GLOBAL, 0, 0};
   L.nb = i;
   return L;
@@ -353,11 +555,30 @@ static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int
     k_move_group<M, AT, S_, LG_><<<grid, 256, 0, st>>>(A, perm, nb);                    \
     return 0;                                                                           \
   }
+#define O_(S_, LG_)                                                                     \
+  if (bs.s == S_ && bs.size == LG_ && bs.upper <= (int64_t)(1 << LG_)) {                \
+    k_move_group_own<M, AT, S_, LG_, (1 << LG_) / S_><<<grid, 256, 0, st>>>(A, perm, nb); \
+    return 0;                                                                           \
+  }
+    O_(4, 3) O_(4, 4) O_(8, 4) O_(8, 5) O_(16, 5) O_(16, 6) O_(32, 6) O_(32, 7)
+    O_(64, 7) O_(64, 8) O_(64, 9) O_(64, 10)
     G_(4, 3) G_(4, 4) G_(8, 4) G_(8, 5) G_(16, 5) G_(16, 6) G_(32, 6) G_(32, 7)
     G_(64, 7) G_(64, 8) G_(64, 9) G_(64, 10)
+#undef O_
 #undef G_
   } else if (bs.shape == SHAPE_BLOCK) {
     const uint32_t H = (uint32_t)bs.size;
+    const size_t lds_own = ((4 * (size_t)H + 15) / 16) * 16 + sizeof(ST) * (size_t)H;
+    const int64_t it = (bs.upper + bs.s - 1) / bs.s;
+#define B_(BS_, IT_)                                                                        \
+  if (bs.s == BS_ && it <= IT_) {                                                           \
+    k_move_block_own<M, AT, BS_, IT_><<<(unsigned)nb, BS_, lds_own, st>>>(A, perm, nb, H);  \
+    return 0;                                                                               \
+  }
+    B_(256, 1) B_(256, 2) B_(256, 4) B_(256, 8) B_(256, 16)
+    B_(512, 1) B_(512, 2) B_(512, 4) B_(512, 8) B_(512, 16)
+    B_(1024, 1) B_(1024, 2) B_(1024, 4) B_(1024, 8) B_(1024, 16)
+#undef B_
     const size_t lds = tab_bytes(H, sizeof(ST), sizeof(AT));
     if (bs.s == 256) {
       k_move_block<M, AT, 256><<<(unsigned)nb, 256, lds, st>>>(A, perm, nb, H);
@@ -393,10 +614,18 @@ void move_kernels_allow_lds(int bytes) {
 #define A_(M_, AT_, BS_) \
   (void)hipFuncSetAttribute((const void*)k_move_block<M_, AT_, BS_>, \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-#define B_(M_, AT_) A_(M_, AT_, 256) A_(M_, AT_, 512) A_(M_, AT_, 1024)
+#define O_(M_, AT_, BS_, IT_) \
+  (void)hipFuncSetAttribute((const void*)k_move_block_own<M_, AT_, BS_, IT_>, \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+#define P_(M_, AT_, BS_) O_(M_, AT_, BS_, 1) O_(M_, AT_, BS_, 2) O_(M_, AT_, BS_, 4) \
+  O_(M_, AT_, BS_, 8) O_(M_, AT_, BS_, 16)
+#define B_(M_, AT_) A_(M_, AT_, 256) A_(M_, AT_, 512) A_(M_, AT_, 1024) \
+  P_(M_, AT_, 256) P_(M_, AT_, 512) P_(M_, AT_, 1024)
   B_(ModeUnit, int32_t) B_(ModeUnit, int64_t) B_(ModeInt, int64_t) B_(ModeF64, double)
   B_(ModeF32, double)
 #undef B_
+#undef P_
+#undef O_
 #undef A_
 }
 
