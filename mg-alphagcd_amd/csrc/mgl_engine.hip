@@ -1735,6 +1735,8 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
     } else if (b.shape == SHAPE_BLOCK) {
       if (b.size < 1 || b.upper > b.size || (b.s != 256 && b.s != 512 && b.s != 1024))
         FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table too small", path, i);
+    } else if (b.shape == SHAPE_LANE) {
+      if (b.upper > 16) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: lane bin above 16", path);
     } else if (b.shape != SHAPE_GLOBAL) {
       FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d shape", path, i);
     }

@@ -10,7 +10,8 @@
 //   SHAPE_BLOCK : one workgroup of BS threads per vertex, LDS hash of `size` slots
 //   SHAPE_GLOBAL: hub path (mgl_hub.hip): rows split into chunks, candidates
 //                 bucketed by hash, one workgroup per bucket
-enum Shape { SHAPE_NONE = 0, SHAPE_GROUP = 1, SHAPE_BLOCK = 2, SHAPE_GLOBAL = 3 };
+//   SHAPE_LANE  : one lane per vertex, the row merged in registers (degree <= 16)
+enum Shape { SHAPE_NONE = 0, SHAPE_GROUP = 1, SHAPE_BLOCK = 2, SHAPE_GLOBAL = 3, SHAPE_LANE = 4 };
 struct BinSpec {
   int64_t upper;  // inclusive degree bound (INT64_MAX for the last bin)
   int shape;

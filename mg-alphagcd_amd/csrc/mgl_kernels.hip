@@ -324,6 +324,34 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
 // The table holds key and partial sum only; after the barrier every creator
 // evaluates its own keys with the total it gathered itself, so neither a
 // table scan nor a second gather is needed.
+// Tables whose sums take 4 bytes keep (sum, key) in one 64-bit word per slot
+// (low half = sum, high half = key): a new key is one 64-bit CAS that also
+// stores its first weight; a known key is one 32-bit add on the low half.
+template <class ST>
+__device__ __forceinline__ int own_packed(uint64_t* T, vid key, ST w, uint32_t H, bool pow2,
+                                          int lg) {
+  uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
+  const uint64_t kk = (uint64_t)(uint32_t)key << 32;
+  while (true) {
+    uint64_t cur = __hip_atomic_load(&T[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if ((uint32_t)(cur >> 32) == (uint32_t)MGL_EMPTY) {
+      uint32_t wb;
+      __builtin_memcpy(&wb, &w, 4);
+      const uint64_t want = kk | wb;
+      const uint64_t old = atomicCAS((unsigned long long*)&T[h],
+                                     (unsigned long long)(((uint64_t)(uint32_t)MGL_EMPTY << 32)),
+                                     (unsigned long long)want);
+      if (old == ((uint64_t)(uint32_t)MGL_EMPTY << 32)) return (int)h;
+      cur = old;
+    }
+    if ((uint32_t)(cur >> 32) == (uint32_t)key) {
+      atom_add((ST*)&T[h], w);  // the low half
+      return -1;
+    }
+    h = pow2 ? ((h + 1u) & ((1u << lg) - 1u)) : ((h + 1u == H) ? 0u : h + 1u);
+  }
+}
+
 template <class ST>
 __device__ __forceinline__ int own_pow2(vid* K, ST* V, vid key, ST w, int lg) {
   const uint32_t mask = (1u << lg) - 1u;
@@ -354,11 +382,48 @@ __device__ __forceinline__ int own_any(vid* K, ST* V, vid key, ST w, uint32_t H)
   }
 }
 
+// Owner-table views: packed (key, 4-byte sum) words, or separate key / sum arrays.
+template <class ST, bool PACK>
+struct OwnTab;
+template <class ST>
+struct OwnTab<ST, true> {
+  uint64_t* T;
+  static constexpr size_t slot = 8;
+  __device__ void init(unsigned char* base, uint32_t) { T = (uint64_t*)base; }
+  __device__ void clear(uint32_t i) { T[i] = (uint64_t)(uint32_t)MGL_EMPTY << 32; }
+  __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg) {
+    return own_packed<ST>(T, key, w, H, pow2, lg);
+  }
+  __device__ vid key(int i) const { return (vid)(uint32_t)(T[i] >> 32); }
+  __device__ ST val(int i) const {
+    const uint32_t lo = (uint32_t)T[i];
+    ST x;
+    __builtin_memcpy(&x, &lo, 4);
+    return x;
+  }
+};
+template <class ST>
+struct OwnTab<ST, false> {
+  vid* K;
+  ST* V;
+  static constexpr size_t slot = 4 + sizeof(ST);
+  __device__ void init(unsigned char* base, uint32_t H) {
+    K = (vid*)base;
+    V = (ST*)(base + ((4 * (size_t)H + 15) / 16) * 16);
+  }
+  __device__ void clear(uint32_t i) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg) {
+    return pow2 ? own_pow2<ST>(K, V, key, w, lg) : own_any<ST>(K, V, key, w, H);
+  }
+  __device__ vid key(int i) const { return K[i]; }
+  __device__ ST val(int i) const { return V[i]; }
+};
+
 // Gathers and inserts the (at most IT per lane) entries of v's row.
-template <class M, class AT, int IT, bool POW2>
+template <class M, class AT, int IT, class TB>
 __device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0, eid e1,
-                                        int lane, int stride, vid* K, typename M::S* V,
-                                        uint32_t tab, int (&own)[IT], AT (&oa)[IT],
+                                        int lane, int stride, TB& tb, uint32_t H, bool pow2,
+                                        int lg, int (&own)[IT], AT (&oa)[IT],
                                         typename M::S& eici, typename M::S& selfw) {
   typedef typename M::S ST;
   const Rec<AT>* R = (const Rec<AT>*)A.rec;
@@ -382,7 +447,7 @@ __device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0
     const vid cj = r[j].c;
     const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
     if (cand) {
-      own[j] = POW2 ? own_pow2<ST>(K, V, cj, wv, (int)tab) : own_any<ST>(K, V, cj, wv, tab);
+      own[j] = tb.ins(cj, wv, H, pow2, lg);
       oa[j] = r[j].a;
     } else if (cj == ci) {
       if (u[j] != v) eici += wv; else selfw += wv;
@@ -390,17 +455,16 @@ __device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0
   }
 }
 
-template <class M, class AT, int IT>
-__device__ __forceinline__ void own_gains(const vid* K, const typename M::S* V,
-                                          const int (&own)[IT], const AT (&oa)[IT],
+template <class M, class AT, int IT, class TB>
+__device__ __forceinline__ void own_gains(const TB& tb, const int (&own)[IT], const AT (&oa)[IT],
                                           double deici, double dki, double daci, double Mh,
                                           bool small, double& best, vid& dst, int& cnt) {
 #pragma unroll
   for (int j = 0; j < IT; ++j) {
     if (own[j] < 0) continue;
     ++cnt;
-    const vid cj = K[own[j]];
-    const double wij = gain((double)V[own[j]], deici, dki, daci, (double)oa[j], Mh);
+    const vid cj = tb.key(own[j]);
+    const double wij = gain((double)tb.val(own[j]), deici, dki, daci, (double)oa[j], Mh);
     if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
   }
 }
@@ -410,17 +474,18 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
                                                         int64_t nb) {
   typedef typename M::S ST;
   typedef typename M::K KT;
+  constexpr bool PACK = sizeof(ST) == 4;
+  typedef OwnTab<ST, PACK> TB;
   constexpr int GPB = 256 / S;
   constexpr int H = 1 << LG;
-  __shared__ vid sk[GPB * H];
-  __shared__ ST sv[GPB * H];
+  __shared__ __align__(16) unsigned char tabs[GPB * H * TB::slot + 16 * GPB];
   __shared__ unsigned long long sdist;
   const int grp = threadIdx.x / S, lane = threadIdx.x & (S - 1);
   const int64_t gi = (int64_t)blockIdx.x * GPB + grp;
-  vid* K = sk + grp * H;
-  ST* V = sv + grp * H;
+  TB tb;
+  tb.init(tabs + (size_t)grp * (H * TB::slot + 16), H);
 #pragma unroll
-  for (int i = lane; i < H; i += S) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  for (int i = lane; i < H; i += S) tb.clear(i);
   if (threadIdx.x == 0) sdist = 0;
   vid v = 0;
   int g = 0;
@@ -438,8 +503,8 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
     const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
     ci = rv.c;
     acik = (KT)rv.a;
-    own_row<M, AT, IT, true>(A, v, ci, A.off[v], A.off[v + 1], lane, S, K, V, LG, own, oa,
-                             eici, selfw);
+    own_row<M, AT, IT, TB>(A, v, ci, A.off[v], A.off[v + 1], lane, S, tb, H, true, LG, own, oa,
+                           eici, selfw);
   }
   __syncthreads();
   if (ok) {
@@ -452,8 +517,8 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
     double best = 0.0;
     vid dst = ci;
     int cnt = 0;
-    own_gains<M, AT, IT>(K, V, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small, best,
-                         dst, cnt);
+    own_gains<M, AT, IT, TB>(tb, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small,
+                             best, dst, cnt);
     group_argmax<S>(best, dst, small);
     if (A.stats) {
       cnt = group_sum<S>(cnt);
@@ -475,6 +540,8 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
                                                        int64_t nb, uint32_t H) {
   typedef typename M::S ST;
   typedef typename M::K KT;
+  constexpr bool PACK = sizeof(ST) == 4;
+  typedef OwnTab<ST, PACK> TB;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double rg[BS / 64];
   __shared__ vid rd[BS / 64];
@@ -485,17 +552,17 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   const vid v = perm[gi];
   int g;
   if (!vertex_active<M>(A, v, g)) return;  // uniform over the workgroup
-  vid* K = (vid*)smem;
-  ST* V = (ST*)(smem + ((4 * (size_t)H + 15) / 16) * 16);
-  for (uint32_t i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
+  TB tb;
+  tb.init(smem, H);
+  for (uint32_t i = threadIdx.x; i < H; i += BS) tb.clear(i);
   const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
   const vid ci = rv.c;
   __syncthreads();
   ST eici = 0, selfw = 0;
   int own[IT];
   AT oa[IT];
-  own_row<M, AT, IT, false>(A, v, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, K, V, H, own, oa,
-                            eici, selfw);
+  own_row<M, AT, IT, TB>(A, v, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, tb, H, false, 0, own,
+                         oa, eici, selfw);
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
   selfw = block_sum<BS>(selfw, rs);
@@ -506,8 +573,8 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   double best = 0.0;
   vid dst = ci;
   int cnt = 0;
-  own_gains<M, AT, IT>(K, V, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small, best,
-                       dst, cnt);
+  own_gains<M, AT, IT, TB>(tb, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small, best,
+                           dst, cnt);
   block_argmax<BS>(best, dst, small, rg, rd);
   if (A.stats) cnt = block_sum<BS>(cnt, rc);
   if (threadIdx.x == 0) {
@@ -518,12 +585,96 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
 }
 
 // ---------------------------------------------------------------------------
+// SHAPE_LANE: one lane per vertex for rows of at most D entries. The row is
+// gathered into registers and equal communities are merged by pairwise
+// comparison (the first occurrence accumulates), so no LDS, barrier or
+// cross-lane operation is involved.
+template <class M, class AT, int D>
+__global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __restrict__ perm,
+                                                   int64_t nb) {
+  typedef typename M::S ST;
+  typedef typename M::K KT;
+  __shared__ unsigned long long sdist;
+  if (threadIdx.x == 0) sdist = 0;
+  if (A.stats) __syncthreads();
+  const int64_t gi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  int g = 0;
+  vid v = 0;
+  bool ok = gi < nb;
+  if (ok) { v = perm[gi]; ok = vertex_active<M>(A, v, g); }
+  int cnt = 0;
+  if (ok) {
+    const Rec<AT>* R = (const Rec<AT>*)A.rec;
+    const Rec<AT> rv = R[v];
+    const vid ci = rv.c;
+    const eid e0 = A.off[v];
+    const int d = (int)(A.off[v + 1] - e0);
+    vid u[D];
+    Rec<AT> r[D];
+    ST w[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) u[j] = j < d ? A.col[e0 + j] : MGL_EMPTY;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
+      w[j] = (u[j] != MGL_EMPTY) ? wload<M>(A.w, e0 + j) : (ST)0;
+    }
+    const bool v1 = A.version == 1;
+    ST eici = 0, selfw = 0;
+    bool cand[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      cand[j] = false;
+      if (u[j] == MGL_EMPTY) continue;
+      const vid cj = r[j].c;
+      cand[j] = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
+      if (!cand[j] && cj == ci) {
+        if (u[j] != v) eici += w[j]; else selfw += w[j];
+      }
+    }
+    // merge duplicates into their first occurrence
+#pragma unroll
+    for (int j = 1; j < D; ++j) {
+#pragma unroll
+      for (int i = 0; i < j; ++i) {
+        if (cand[i] && cand[j] && r[i].c == r[j].c) {
+          w[i] += w[j];
+          cand[j] = false;
+        }
+      }
+    }
+    const double Mh = A.gs[g].mass / 2.0;
+    const KT ki = ((const KT*)A.k)[v];
+    const KT aci_k = (KT)rv.a - ki;
+    const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
+    const bool small = A.tie_small ? true : (A.up != 0);
+    double best = 0.0;
+    vid dst = ci;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      if (!cand[j]) continue;
+      ++cnt;
+      const double wij = gain((double)w[j], deici, dki, daci, (double)r[j].a, Mh);
+      if (better(wij, r[j].c, best, dst, small)) { best = wij; dst = r[j].c; }
+    }
+    A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
+    if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
+  }
+  if (A.stats) {
+    cnt = group_sum<64>(cnt);
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&sdist, (unsigned long long)cnt);
+    __syncthreads();
+    if (threadIdx.x == 0 && sdist) atomicAdd(A.stats, sdist);
+  }
+}
+
+// ---------------------------------------------------------------------------
 BinLayout default_bins(int lds_bytes, int slot_bytes) {
   BinLayout L;
   int i = 0;
   L.b[i++] = {0, SHAPE_NONE, 0, 0};
-  L.b[i++] = {4, SHAPE_GROUP, 4, 3};
-  L.b[i++] = {8, SHAPE_GROUP, 8, 4};
+  L.b[i++] = {4, SHAPE_LANE, 1, 0};
+  L.b[i++] = {8, SHAPE_LANE, 1, 0};
   L.b[i++] = {16, SHAPE_GROUP, 16, 5};
   L.b[i++] = {32, SHAPE_GROUP, 32, 6};
   L.b[i++] = {64, SHAPE_GROUP, 64, 7};
@@ -547,6 +698,13 @@ static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int
                       hipStream_t st) {
   typedef typename M::S ST;
   if (nb <= 0 || bs.shape == SHAPE_NONE) return 0;
+  if (bs.shape == SHAPE_LANE) {
+    const unsigned grid = (unsigned)((nb + 255) / 256);
+    if (bs.upper <= 4) { k_move_lane<M, AT, 4><<<grid, 256, 0, st>>>(A, perm, nb); return 0; }
+    if (bs.upper <= 8) { k_move_lane<M, AT, 8><<<grid, 256, 0, st>>>(A, perm, nb); return 0; }
+    if (bs.upper <= 16) { k_move_lane<M, AT, 16><<<grid, 256, 0, st>>>(A, perm, nb); return 0; }
+    return -1;
+  }
   if (bs.shape == SHAPE_GROUP) {
     const int gpb = 256 / bs.s;
     const unsigned grid = (unsigned)((nb + gpb - 1) / gpb);
@@ -568,7 +726,8 @@ static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int
 #undef G_
   } else if (bs.shape == SHAPE_BLOCK) {
     const uint32_t H = (uint32_t)bs.size;
-    const size_t lds_own = ((4 * (size_t)H + 15) / 16) * 16 + sizeof(ST) * (size_t)H;
+    const size_t lds_own = (sizeof(ST) == 4) ? 8 * (size_t)H
+                                             : ((4 * (size_t)H + 15) / 16) * 16 + sizeof(ST) * (size_t)H;
     const int64_t it = (bs.upper + bs.s - 1) / bs.s;
 #define B_(BS_, IT_)                                                                        \
   if (bs.s == BS_ && it <= IT_) {                                                           \
@@ -686,11 +845,11 @@ template <class M, int OP>
 static void rows_bin_t(const BinSpec& bs, const RowArgs& A, const vid* perm, int64_t nb,
                        hipStream_t st) {
   if (nb <= 0 || bs.shape == SHAPE_NONE) return;
-  if (bs.shape == SHAPE_GROUP && bs.upper <= 4) {
+  if (bs.upper <= 4) {
     k_rows<M, 4, OP><<<(unsigned)((nb + 63) / 64), 256, 0, st>>>(A, perm, nb);
-  } else if (bs.shape == SHAPE_GROUP && bs.upper <= 16) {
+  } else if (bs.upper <= 16) {
     k_rows<M, 16, OP><<<(unsigned)((nb + 15) / 16), 256, 0, st>>>(A, perm, nb);
-  } else if (bs.shape == SHAPE_GROUP) {
+  } else if (bs.upper <= 512) {
     k_rows<M, 64, OP><<<(unsigned)((nb + 3) / 4), 256, 0, st>>>(A, perm, nb);
   } else {
     k_rows<M, 256, OP><<<(unsigned)nb, 256, 0, st>>>(A, perm, nb);
