@@ -99,6 +99,9 @@ def main():
     ap.add_argument("--full", action="store_true", help="also time the full multi-level run")
     ap.add_argument("--tuning-file", default=None)
     ap.add_argument("--stats-out", default=None, help="write per-pass statistics (JSON)")
+    ap.add_argument("--autotune", default=None,
+                    help="run the bin-layout autotuner on this graph, write the layout to "
+                         "this path and time with it")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -114,6 +117,11 @@ def main():
 
     ctx = mgl.Context(0)
     log(f"device LDS per workgroup: {ctx.lds_bytes()} bytes; bins: {ctx.bin_bounds()}")
+    if args.autotune:
+        t0 = time.perf_counter()
+        ctx.autotune(g, args.autotune)
+        args.tuning_file = args.autotune
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): " + open(args.autotune).read())
     sess = ctx.session(g, version=args.version, fuse_modularity=not args.no_fuse,
                        tuning_file=args.tuning_file)
     bins = sess.bins()

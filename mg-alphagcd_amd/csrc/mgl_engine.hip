@@ -1768,18 +1768,25 @@ static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
       if (D.b[i].upper > top) L.b[L.nb++] = D.b[i];
     return L;
   };
-  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {8, SHAPE_GROUP, 4, 4}, {16, SHAPE_GROUP, 8, 5},
-                      {32, SHAPE_GROUP, 16, 6}, {64, SHAPE_GROUP, 32, 7},
-                      {128, SHAPE_GROUP, 64, 8}, {512, SHAPE_GROUP, 64, 10},
+  // one lane per vertex up to 16
+  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_LANE, 1, 0}, {8, SHAPE_LANE, 1, 0},
+                      {16, SHAPE_LANE, 1, 0}}));
+  // more lanes per vertex (one entry each)
+  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_LANE, 1, 0}, {8, SHAPE_LANE, 1, 0},
+                      {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6},
+                      {64, SHAPE_GROUP, 64, 7}, {128, SHAPE_GROUP, 64, 8},
+                      {256, SHAPE_GROUP, 64, 9}}));
+  // fewer lanes per vertex (four entries each) further up, wavefront up to 512
+  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_LANE, 1, 0}, {8, SHAPE_LANE, 1, 0},
+                      {16, SHAPE_GROUP, 4, 5}, {32, SHAPE_GROUP, 8, 6},
+                      {64, SHAPE_GROUP, 16, 7}, {128, SHAPE_GROUP, 32, 8},
+                      {256, SHAPE_GROUP, 64, 9}, {512, SHAPE_GROUP, 64, 10},
                       {2048, SHAPE_BLOCK, 512, 4096}}));
-  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_GROUP, 4, 3}, {8, SHAPE_GROUP, 8, 4},
-                      {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6},
-                      {64, SHAPE_GROUP, 64, 7}, {128, SHAPE_GROUP, 64, 8},
+  // workgroups from degree 129
+  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_LANE, 1, 0}, {8, SHAPE_LANE, 1, 0},
+                      {16, SHAPE_GROUP, 4, 5}, {32, SHAPE_GROUP, 8, 6},
+                      {64, SHAPE_GROUP, 16, 7}, {128, SHAPE_GROUP, 32, 8},
                       {512, SHAPE_BLOCK, 256, 1024}, {4096, SHAPE_BLOCK, 1024, 8192}}));
-  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_GROUP, 4, 3}, {8, SHAPE_GROUP, 8, 4},
-                      {16, SHAPE_GROUP, 16, 5}, {32, SHAPE_GROUP, 32, 6},
-                      {64, SHAPE_GROUP, 64, 7}, {128, SHAPE_GROUP, 64, 8},
-                      {256, SHAPE_GROUP, 64, 9}, {2048, SHAPE_BLOCK, 512, 4096}}));
   return out;
 }
 

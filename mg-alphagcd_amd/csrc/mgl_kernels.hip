@@ -675,10 +675,10 @@ BinLayout default_bins(int lds_bytes, int slot_bytes) {
   L.b[i++] = {0, SHAPE_NONE, 0, 0};
   L.b[i++] = {4, SHAPE_LANE, 1, 0};
   L.b[i++] = {8, SHAPE_LANE, 1, 0};
-  L.b[i++] = {16, SHAPE_GROUP, 16, 5};
-  L.b[i++] = {32, SHAPE_GROUP, 32, 6};
-  L.b[i++] = {64, SHAPE_GROUP, 64, 7};
-  L.b[i++] = {128, SHAPE_GROUP, 64, 8};
+  L.b[i++] = {16, SHAPE_GROUP, 4, 5};
+  L.b[i++] = {32, SHAPE_GROUP, 8, 6};
+  L.b[i++] = {64, SHAPE_GROUP, 16, 7};
+  L.b[i++] = {128, SHAPE_GROUP, 32, 8};
   L.b[i++] = {256, SHAPE_GROUP, 64, 9};
   L.b[i++] = {1024, SHAPE_BLOCK, 256, 2048};
   L.b[i++] = {4096, SHAPE_BLOCK, 1024, 8192};
@@ -713,13 +713,19 @@ static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int
     k_move_group<M, AT, S_, LG_><<<grid, 256, 0, st>>>(A, perm, nb);                    \
     return 0;                                                                           \
   }
-#define O_(S_, LG_)                                                                     \
-  if (bs.s == S_ && bs.size == LG_ && bs.upper <= (int64_t)(1 << LG_)) {                \
-    k_move_group_own<M, AT, S_, LG_, (1 << LG_) / S_><<<grid, 256, 0, st>>>(A, perm, nb); \
+// owner-based instances (lanes S, table 2^LG, entries per lane IT); the
+// smallest IT that covers the bin is taken, longer rows use the scan kernel
+#define O_(S_, LG_, IT_)                                                                \
+  if (bs.s == S_ && bs.size == LG_ && need <= IT_) {                                    \
+    k_move_group_own<M, AT, S_, LG_, IT_><<<grid, 256, 0, st>>>(A, perm, nb);           \
     return 0;                                                                           \
   }
-    O_(4, 3) O_(4, 4) O_(8, 4) O_(8, 5) O_(16, 5) O_(16, 6) O_(32, 6) O_(32, 7)
-    O_(64, 7) O_(64, 8) O_(64, 9) O_(64, 10)
+    const int64_t need = (bs.upper + bs.s - 1) / bs.s;
+    O_(4, 3, 1) O_(4, 3, 2) O_(4, 4, 4) O_(4, 5, 4)
+    O_(8, 4, 1) O_(8, 5, 2) O_(8, 5, 4) O_(8, 6, 4)
+    O_(16, 5, 1) O_(16, 6, 2) O_(16, 6, 4) O_(16, 7, 4)
+    O_(32, 6, 1) O_(32, 7, 2) O_(32, 7, 4) O_(32, 8, 4)
+    O_(64, 7, 1) O_(64, 8, 2) O_(64, 8, 4) O_(64, 9, 4) O_(64, 9, 8) O_(64, 10, 8)
     G_(4, 3) G_(4, 4) G_(8, 4) G_(8, 5) G_(16, 5) G_(16, 6) G_(32, 6) G_(32, 7)
     G_(64, 7) G_(64, 8) G_(64, 9) G_(64, 10)
 #undef O_
