@@ -1768,6 +1768,12 @@ static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
       if (D.b[i].upper > top) L.b[L.nb++] = D.b[i];
     return L;
   };
+  // workgroups of 256 from degree 257
+  out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_LANE, 1, 0}, {8, SHAPE_LANE, 1, 0},
+                      {16, SHAPE_GROUP, 4, 5}, {32, SHAPE_GROUP, 8, 6},
+                      {64, SHAPE_GROUP, 16, 7}, {128, SHAPE_GROUP, 32, 8},
+                      {256, SHAPE_GROUP, 64, 9}, {1024, SHAPE_BLOCK, 256, 2048},
+                      {4096, SHAPE_BLOCK, 1024, 8192}}));
   // one lane per vertex up to 16
   out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_LANE, 1, 0}, {8, SHAPE_LANE, 1, 0},
                       {16, SHAPE_LANE, 1, 0}}));

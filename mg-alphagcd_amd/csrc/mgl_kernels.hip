@@ -680,7 +680,8 @@ BinLayout default_bins(int lds_bytes, int slot_bytes) {
   L.b[i++] = {64, SHAPE_GROUP, 16, 7};
   L.b[i++] = {128, SHAPE_GROUP, 32, 8};
   L.b[i++] = {256, SHAPE_GROUP, 64, 9};
-  L.b[i++] = {1024, SHAPE_BLOCK, 256, 2048};
+  L.b[i++] = {512, SHAPE_GROUP, 64, 10};
+  L.b[i++] = {2048, SHAPE_BLOCK, 512, 4096};
   L.b[i++] = {4096, SHAPE_BLOCK, 1024, 8192};
   // the largest table the workgroup LDS holds, filled to at most 3/4, and at
   // most 16 entries per thread (owner-based kernel)
