@@ -162,3 +162,28 @@ def coarsen(off, col, w, c):
                              _p(rank, _i32))
     m2 = int(off2[C2])
     return off2[:C2 + 1].copy(), col2[:m2].copy(), w2[:m2].copy(), rank[:n].copy()
+
+
+def modularity_fp64(off, col, w, labels):
+    """Independent FP64 modularity of a partition (sequential double sums):
+    Q = sum_in/2m - sum_c tot_c^2/(2m)^2 with tot_c = sum of row weights."""
+    off = np.ascontiguousarray(off, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    lab = np.ascontiguousarray(labels, np.int32)
+    n = len(off) - 1
+    wd = None if w is None else np.ascontiguousarray(w, np.float64)
+    deg = np.diff(off)
+    if wd is None:
+        k = deg.astype(np.float64)
+        mass = float(len(col))
+    else:
+        k = np.add.reduceat(wd, off[:-1]) if n else np.zeros(0)
+        k[deg == 0] = 0.0
+        mass = float(wd.sum())
+    tot = np.zeros(max(n, 1), np.float64)
+    np.add.at(tot, lab, k)
+    a = np.zeros(max(n, 1), np.float64)
+    a[:n] = tot[:n]
+    return lib.orc_modularity_f64(n, _p(off, _i64), _p(col, _i32),
+                                  None if wd is None else _p(wd, _d), _p(lab, _i32), _p(a, _d),
+                                  mass)
