@@ -177,8 +177,8 @@ def modularity_fp64(off, col, w, labels):
         k = deg.astype(np.float64)
         mass = float(len(col))
     else:
-        k = np.add.reduceat(wd, off[:-1]) if n else np.zeros(0)
-        k[deg == 0] = 0.0
+        rows = np.repeat(np.arange(n, dtype=np.int64), deg)
+        k = np.bincount(rows, weights=wd, minlength=n).astype(np.float64)
         mass = float(wd.sum())
     tot = np.zeros(max(n, 1), np.float64)
     np.add.at(tot, lab, k)
