@@ -197,7 +197,7 @@ struct Engine {
   DBuf cflag, crank, cnewc, ck0, ck1, cv0, cv1, chist, chead, cuidx, crowcnt, cscan;
   CoarsenBufs CB{};
   // level graphs (ping-pong)
-  LevelGraph lg[2];
+  LevelGraph lg[3];  // [0] = input (kept), [1]/[2] = coarse levels in turn
   int cur = 0;
   // host-side phase state per graph
   std::vector<double> Q_host, Q_old;
@@ -630,6 +630,7 @@ struct Engine {
   }
 
   mgl_status level_end(int level);
+  mgl_status final_q_fp64(std::vector<double>& q);
   mgl_status run(std::vector<mgl_result*>& out);
   mgl_status setup_union(const mgl_csr* graphs, int ng);
 };
@@ -654,6 +655,50 @@ mgl_status Engine::recompute_a(const vid* cc) {
   else
     k_scatter_a_d<<<grid, 256, 0, st()>>>(g.n, cc, bk.as<double>(), ba.as<double>());
   HIPC(ctx, hipGetLastError());
+  return MGL_OK;
+}
+
+__global__ void k_global_labels(int64_t n0, const vid* fin, const int32_t* gid0,
+                                const int64_t* vbase, vid* out) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n0;
+       v += (int64_t)gridDim.x * blockDim.x)
+    out[v] = fin[v] + (vid)vbase[gid0 ? gid0[v] : 0];
+}
+
+// FP64 modularity of the final labels on the input graph (float modes).
+mgl_status Engine::final_q_fp64(std::vector<double>& q) {
+  cur = 0;
+  LevelGraph& g = LG();
+  std::vector<int32_t> run(G, 1);
+  HIPC(ctx, hipMemcpyAsync(brun.p, run.data(), 4 * G, hipMemcpyHostToDevice, st()));
+  HIPC(ctx, hipStreamSynchronize(st()));
+  launch_level_begin(G, bgs.as<GState>(), brun.as<int32_t>(), st());
+  mgl_status s = make_bins();
+  if (s) return s;
+  // global label of every input vertex
+  DBuf dv;
+  HIPC(ctx, dv.ensure(8 * (G + 1)));
+  HIPC(ctx, hipMemcpyAsync(dv.p, vbase0.data(), 8 * (G + 1), hipMemcpyHostToDevice, st()));
+  const unsigned gr = (unsigned)std::min<int64_t>(8192, (n0 + 255) / 256 + 1);
+  k_global_labels<<<gr, 256, 0, st()>>>(n0, bfin.as<vid>(), bgid0.as<int32_t>(), dv.as<int64_t>(),
+                                        c);
+  launch_fill_bytes0(bk.p, 8 * (size_t)std::max<int64_t>(g.n, 1), st());
+  launch_fill_bytes0(bvin.p, 8 * (size_t)std::max<int64_t>(g.n, 1), st());
+  s = rows(OP_WEIGHT, nullptr, bk.p, bvin.p, false);
+  if (s) return s;
+  s = recompute_a(c);
+  if (s) return s;
+  launch_fill_bytes0(bvin.p, 8 * (size_t)std::max<int64_t>(g.n, 1), st());
+  s = rows(OP_INTRA, c, nullptr, bvin.p, false);
+  if (s) return s;
+  launch_reduce(g.mode, g.n, bvin.p, ba.p, g.gid.as<int32_t>(), bgs.as<GState>(), st());
+  s = read_gs();
+  if (s) return s;
+  q.resize(G);
+  for (int i = 0; i < G; ++i) {
+    const double M = hgs[i].mass;
+    q[i] = (M == 0) ? 0.0 : hgs[i].sin_d / M - hgs[i].sa2_d / (M * M);
+  }
   return MGL_OK;
 }
 
@@ -816,7 +861,8 @@ mgl_status Engine::level_end(int level) {
   }
   HIPC(ctx, hipMemcpyAsync(bdelta.p, delta.data(), 8 * G, hipMemcpyHostToDevice, st()));
   HIPC(ctx, hipStreamSynchronize(st()));
-  LevelGraph& ng = lg[cur ^ 1];
+  const int nxt = (cur == 1) ? 2 : 1;
+  LevelGraph& ng = lg[nxt];
   ng.mode = coarse_mode(g.mode);
   // upper bounds: C <= n, E2 <= e_keep
   HIPC(ctx, ng.off.ensure(8 * (g.n + 2)));
@@ -864,7 +910,7 @@ mgl_status Engine::level_end(int level) {
   k_label_next<<<g0, 256, 0, st()>>>(n0, blab.as<vid>(), CB.newc, bgid0.as<int32_t>(),
                                      bkeep.as<int32_t>());
   HIPC(ctx, hipGetLastError());
-  cur ^= 1;
+  cur = nxt;
   return MGL_OK;
 }
 
@@ -895,6 +941,13 @@ mgl_status Engine::run(std::vector<mgl_result*>& out) {
     if (s) return s;
     ++level;
   }
+  // float modes: the reported final modularity is an FP64 reduction over the
+  // input graph with the final labels
+  std::vector<double> qfin;
+  if (lg[0].mode == MODE_F32 || lg[0].mode == MODE_F64) {
+    mgl_status s = final_q_fp64(qfin);
+    if (s) return s;
+  }
   // final labels
   std::vector<int32_t> fin(n0);
   if (n0) {
@@ -909,7 +962,7 @@ mgl_status Engine::run(std::vector<mgl_result*>& out) {
     mgl_result* r = out[i];
     const int64_t b = vbase0[i], e = vbase0[i + 1];
     r->final_assign.assign(fin.begin() + b, fin.begin() + e);
-    r->final_q = r->levels.empty() ? 0.0 : r->levels.back().q;
+    r->final_q = r->levels.empty() ? 0.0 : (qfin.empty() ? r->levels.back().q : qfin[i]);
     r->seconds = secs;
     r->precision = prec;
   }
