@@ -786,9 +786,14 @@ mgl_status Engine::level_end(int level) {
   LevelGraph& g = LG();
   mgl_status s = read_gs();
   if (s) return s;
+  // trajectories: only the columns any graph used
+  int tmax = 1;
+  for (int i = 0; i < G; ++i)
+    if (running[i]) tmax = std::max(tmax, std::min(hgs[i].iters, TRAJ_MAX));
   std::vector<double> traj((size_t)G * TRAJ_MAX);
-  HIPC(ctx, hipMemcpyAsync(traj.data(), btraj.p, sizeof(double) * traj.size(),
-                           hipMemcpyDeviceToHost, st()));
+  HIPC(ctx, hipMemcpy2DAsync(traj.data(), sizeof(double) * TRAJ_MAX, btraj.p,
+                             sizeof(double) * TRAJ_MAX, sizeof(double) * tmax, G,
+                             hipMemcpyDeviceToHost, st()));
   std::vector<int32_t> hc;
   if (o.keep_levels && g.n > 0) {
     hc.resize(g.n);
