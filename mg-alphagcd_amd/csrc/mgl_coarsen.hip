@@ -286,24 +286,66 @@ __global__ void k_heads(const uint64_t* __restrict__ k, int64_t n, int32_t* __re
     head[i] = (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
 }
 
+// Coarse entries from the sorted keys. Equal keys are contiguous, so each
+// wavefront sums the runs it holds with a segmented scan and the last lane of
+// a run adds the run's sum: about one global atomic per distinct key (plus
+// one per wavefront boundary inside long runs) instead of one per entry.
+template <class T>
+__device__ __forceinline__ T shfl_up_t(T v, int d) {
+  if constexpr (sizeof(T) == 8) {
+    uint64_t x;
+    __builtin_memcpy(&x, &v, 8);
+    int lo = __shfl_up((int)(uint32_t)x, d, 64);
+    int hi = __shfl_up((int)(uint32_t)(x >> 32), d, 64);
+    uint64_t y = ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+    T r;
+    __builtin_memcpy(&r, &y, 8);
+    return r;
+  } else {
+    return __shfl_up(v, d, 64);
+  }
+}
+
 template <class VT, class OT, bool HAS_V>
-__global__ void k_emit(const uint64_t* __restrict__ k, const VT* __restrict__ v, int64_t n,
-                       const int32_t* __restrict__ head, const int64_t* __restrict__ uidx,
-                       int bits, vid* __restrict__ col2, OT* __restrict__ w2,
-                       int64_t* __restrict__ rowcnt) {
+__global__ __launch_bounds__(256) void k_emit(const uint64_t* __restrict__ k,
+                                              const VT* __restrict__ v, int64_t n,
+                                              const int32_t* __restrict__ head,
+                                              const int64_t* __restrict__ uidx, int bits,
+                                              vid* __restrict__ col2, OT* __restrict__ w2,
+                                              int64_t* __restrict__ rowcnt) {
   const uint64_t mask = (bits >= 64) ? ~0ull : ((1ull << bits) - 1ull);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t u = uidx[i + 1] - 1;
-    if (head[i]) {
-      col2[u] = (vid)(k[i] & mask);
-      atomicAdd((unsigned long long*)&rowcnt[k[i] >> bits], 1ull);
+  const int lane = threadIdx.x & 63;
+  for (int64_t base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63); base < n;
+       base += (int64_t)gridDim.x * 256) {
+    const int64_t i = base + lane;
+    const bool ok = i < n;
+    const uint64_t key = ok ? k[i] : ~0ull;
+    OT x = ok ? (HAS_V ? (OT)v[i] : (OT)1) : (OT)0;
+    int64_t u = ok ? uidx[i + 1] - 1 : -1;
+    if (ok && head[i]) {
+      col2[u] = (vid)(key & mask);
+      atomicAdd((unsigned long long*)&rowcnt[key >> bits], 1ull);
     }
-    const OT x = HAS_V ? (OT)v[i] : (OT)1;
-    if constexpr (std::is_same_v<OT, int64_t>)
-      atomicAdd((unsigned long long*)&w2[u], (unsigned long long)x);
-    else
-      atomicAdd(&w2[u], x);
+    // segmented inclusive scan over equal u within the wavefront
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const OT y = shfl_up_t(x, d);
+      const int64_t uy = shfl_up_t(u, d);
+      if (lane >= d && uy == u) x += y;
+    }
+    int64_t u_next;
+    {
+      int lo = __shfl_down((int)(uint32_t)(uint64_t)u, 1, 64);
+      int hi = __shfl_down((int)(uint32_t)((uint64_t)u >> 32), 1, 64);
+      u_next = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+    const bool last = ok && (lane == 63 || u_next != u || base + lane + 1 >= n);
+    if (last) {
+      if constexpr (std::is_same_v<OT, int64_t>)
+        atomicAdd((unsigned long long*)&w2[u], (unsigned long long)x);
+      else
+        atomicAdd(&w2[u], x);
+    }
   }
 }
 
@@ -346,7 +388,7 @@ static int coarsen_t(int64_t n, const eid* off, const vid* col, const void* w, c
   launch_fill_bytes0(B.rowcnt, sizeof(int64_t) * (size_t)(C + 1), st);
   launch_fill_bytes0(w2, sizeof(OT) * (size_t)(U > 0 ? U : 1), st);
   if (e_keep > 0)
-    k_emit<VT, OT, HAS_V><<<gridc(e_keep, 256, 8192), 256, 0, st>>>(ks, vs, e_keep, B.head,
+    k_emit<VT, OT, HAS_V><<<gridc(e_keep, 256, 16384), 256, 0, st>>>(ks, vs, e_keep, B.head,
                                                                    B.uidx, bits, col2, (OT*)w2,
                                                                    B.rowcnt);
   scan_exclusive_i64(B.rowcnt, off2, C, B.scan_tmp, st);

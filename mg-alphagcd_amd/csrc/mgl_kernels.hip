@@ -1094,14 +1094,30 @@ __global__ __launch_bounds__(256) void k_reduce(int64_t n, const KT* __restrict_
   acc.flush(&gs[gcur]);
 }
 
+// One graph: grid-stride partials, one flush per wavefront.
+template <class KT, bool EXACT>
+__global__ __launch_bounds__(256) void k_reduce1(int64_t n, const KT* __restrict__ vin,
+                                                 const KT* __restrict__ a, GState* gs) {
+  if (!gs[0].active) return;
+  RedAcc<EXACT> acc;
+  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256)
+    acc.add(vin ? vin[v] : (KT)0, a[v]);
+  acc.flush(&gs[0]);
+}
+
 void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const int32_t* gid,
                    GState* gs, hipStream_t st) {
   if (n <= 0) return;
-  int64_t waves = (n + 1023) / 1024;  // >= 1024 vertices per wave
-  if (waves > 4096) waves = 4096;
-  int64_t per_wave = (n + waves - 1) / waves;
-  per_wave = (per_wave + 63) / 64 * 64;
-  waves = (n + per_wave - 1) / per_wave;
+  if (!gid) {
+    const unsigned grid = grid_for(n, 256 * 8, 1024);
+    if (mode == MODE_UNIT || mode == MODE_INT)
+      k_reduce1<int64_t, true><<<grid, 256, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a, gs);
+    else
+      k_reduce1<double, false><<<grid, 256, 0, st>>>(n, (const double*)vin, (const double*)a, gs);
+    return;
+  }
+  const int64_t per_wave = 256;
+  const int64_t waves = (n + per_wave - 1) / per_wave;
   const unsigned grid = (unsigned)((waves + 3) / 4);
   if (mode == MODE_UNIT || mode == MODE_INT)
     k_reduce<int64_t, true><<<grid, 256, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a,
