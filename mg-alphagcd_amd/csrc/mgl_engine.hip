@@ -181,7 +181,8 @@ struct Engine {
   int64_t n0 = 0, nnz0 = 0;
   std::vector<int64_t> vbase0;  // per-graph level-0 vertex ranges [G+1]
   // per-vertex state
-  DBuf bc, bp, bn, bk, ba, bvin, bperm, blab, bfin, bgid0, brec;
+  DBuf bc, bp, bn, bk, ba, bvin, bperm, blab, bfin, bgid0, brec, bperm2, blist, bcnt2;
+  int64_t nlist = -1;  // >= 0: per-vertex passes walk blist (active vertices of a batch)
   bool rec32 = false;  // 8-byte records (unit mode, every graph's 2m < 2^31)
   vid* c = nullptr; vid* p = nullptr; vid* nx = nullptr;
   // per-graph
@@ -218,6 +219,8 @@ struct Engine {
   }
   hipStream_t st() const { return ctx->st; }
   bool rec32_now() { return rec32 && LG().mode == MODE_UNIT; }
+  const vid* vlist() const { return nlist >= 0 ? blist.as<vid>() : nullptr; }
+  int64_t vcount() { return nlist >= 0 ? nlist : LG().n; }
   // device word: graphs still active after the last decision
   int* nactive() { return (int*)(bcounts.as<unsigned long long>() + 3 * MGL_MAX_BINS); }
   LevelGraph& LG() { return lg[cur]; }
@@ -233,6 +236,11 @@ struct Engine {
     HIPC(ctx, bvin.ensure(8 * n));
     HIPC(ctx, bperm.ensure(4 * n));
     HIPC(ctx, brec.ensure(16 * n));
+    if (G > 1) {
+      HIPC(ctx, bperm2.ensure(4 * n));
+      HIPC(ctx, blist.ensure(4 * n));
+      HIPC(ctx, bcnt2.ensure(8 * MGL_MAX_BINS));
+    }
     HIPC(ctx, blab.ensure(4 * n));
     HIPC(ctx, bfin.ensure(4 * n));
     HIPC(ctx, bgs.ensure(sizeof(GState) * G));
@@ -426,7 +434,7 @@ struct Engine {
     LevelGraph& g = LG();
     MoveArgs A = move_args(from, to, up, vin);
     const int r32 = rec32_now() ? 1 : 0;
-    launch_build_rec(g.mode, r32, g.n, from, ba.p, brec.p, st());
+    launch_build_rec(g.mode, r32, vcount(), from, ba.p, brec.p, st(), vlist());
     int ne = 0;
     for (int b = L.nb - 1; b >= 1; --b) {
       if (bin_n[b] == 0) continue;
@@ -473,6 +481,7 @@ struct Engine {
     HIPC(ctx, hipMemcpyAsync(brun.p, run.data(), 4 * G, hipMemcpyHostToDevice, st()));
     HIPC(ctx, hipStreamSynchronize(st()));
     launch_level_begin(G, bgs.as<GState>(), brun.as<int32_t>(), st());
+    nlist = -1;
     mgl_status s = make_bins();
     if (s) return s;
     launch_fill_bytes0(bk.p, 8 * (size_t)std::max<int64_t>(g.n, 1), st());
@@ -515,28 +524,28 @@ struct Engine {
       mgl_status s = move(p, nx, up, bvin.p, timing);
       if (s) return s;
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 1], st()));
-      launch_update(g.mode, g.n, p, nx, bk.p, ba.p, gid, gs, st());
+      launch_update(g.mode, vcount(), p, nx, bk.p, ba.p, gid, gs, st(), vlist());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 2], st()));
-      launch_reduce(g.mode, g.n, bvin.p, ba.p, gid, gs, st());
+      launch_reduce(g.mode, vcount(), bvin.p, ba.p, gid, gs, st(), vlist());
       launch_fill_bytes0(nactive(), 4, st());
       launch_decide(G, gs, o.threshold, o.max_iter, 1, ex, traj, TRAJ_MAX, 0, st(), nactive());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 3], st()));
-      launch_fixup(g.n, c, p, nx, gid, gs, st());
+      launch_fixup(vcount(), c, p, nx, gid, gs, st(), vlist());
       vid* t = c; c = p; p = nx; nx = t;
     } else {
       const int up = (it & 1) ? 1 : 0;
       mgl_status s = move(c, p, up, nullptr, timing);
       if (s) return s;
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 1], st()));
-      launch_update(g.mode, g.n, c, p, bk.p, ba.p, gid, gs, st());
+      launch_update(g.mode, vcount(), c, p, bk.p, ba.p, gid, gs, st(), vlist());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 2], st()));
       s = rows(OP_INTRA, p, nullptr, bvin.p, true);
       if (s) return s;
-      launch_reduce(g.mode, g.n, bvin.p, ba.p, gid, gs, st());
+      launch_reduce(g.mode, vcount(), bvin.p, ba.p, gid, gs, st(), vlist());
       launch_fill_bytes0(nactive(), 4, st());
       launch_decide(G, gs, o.threshold, o.max_iter, 0, ex, traj, TRAJ_MAX, 0, st(), nactive());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 3], st()));
-      launch_fixup(g.n, c, p, nullptr, gid, gs, st());
+      launch_fixup(vcount(), c, p, nullptr, gid, gs, st(), vlist());
     }
     if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 4], st()));
     HIPC(ctx, hipGetLastError());
@@ -564,17 +573,48 @@ struct Engine {
     return MGL_OK;
   }
 
+  // Batch compaction: keep only the vertices of graphs still active in the
+  // move bins, and list the active non-isolated vertices (increasing ids) for
+  // the per-vertex passes, so finished graphs stop costing launch work.
+  mgl_status compact() {
+    LevelGraph& g = LG();
+    unsigned long long* cnt = bcnt2.as<unsigned long long>();
+    launch_fill_bytes0(cnt, 8 * MGL_MAX_BINS, st());
+    for (int b = 1; b < L.nb; ++b)
+      if (bin_n[b] > 0)
+        launch_filter_bin(bperm.as<vid>() + bin_off[b], bin_n[b], g.gid.as<int32_t>(),
+                          bgs.as<GState>(), bperm2.as<vid>() + bin_off[b], cnt + b, st());
+    launch_active_list(g.n, g.off.as<eid>(), g.gid.as<int32_t>(), bgs.as<GState>(), CB.flag,
+                       CB.rank, blist.as<vid>(), CB.scan_tmp, st());
+    std::vector<unsigned long long> h(MGL_MAX_BINS);
+    HIPC(ctx, hipMemcpyAsync(h.data(), cnt, 8 * MGL_MAX_BINS, hipMemcpyDeviceToHost, st()));
+    int64_t nl = 0;
+    HIPC(ctx, hipMemcpyAsync(&nl, CB.rank + g.n, 8, hipMemcpyDeviceToHost, st()));
+    HIPC(ctx, hipStreamSynchronize(st()));
+    for (int b = 1; b < L.nb; ++b) bin_n[b] = (int64_t)h[b];
+    std::swap(bperm.p, bperm2.p);
+    std::swap(bperm.bytes, bperm2.bytes);
+    nlist = nl;
+    HIPC(ctx, hipGetLastError());
+    return MGL_OK;
+  }
+
   // Inner loop of a level (louvain.cu:1548-1660). With use_graph the
   // iterations are captured once per level into a hipGraph holding R of them
   // (R = 6: the proposal buffers rotate with period 3 and the up/down
   // direction alternates with period 2) and replayed until no graph of the
-  // batch is active; iterations after a graph's last decision are no-ops for it.
+  // batch is active; iterations after a graph's last decision are no-ops for
+  // it. In a batch, whenever the number of active graphs has halved the
+  // vertex sets are compacted (and the hipGraph captured again).
   mgl_status run_level() {
     mgl_status s;
     if (o.fuse_modularity) {
       s = pre_move();
       if (s) return s;
     }
+    int last_na = 0;
+    for (int i = 0; i < G; ++i) last_na += running[i] ? 1 : 0;
+    const bool can_compact = G > 1 && CB.flag != nullptr;
     if (!o.use_graph) {
       for (int it = 1;; ++it) {
         if (it > (1 << 22)) FAIL(ctx, MGL_DEVICE_ERROR, "inner loop did not terminate");
@@ -584,49 +624,70 @@ struct Engine {
         s = read_nactive(na);
         if (s) return s;
         if (na == 0) break;
+        if (can_compact && 2 * na <= last_na) {
+          s = compact();
+          if (s) return s;
+          last_na = na;
+        }
       }
       return MGL_OK;
     }
     const int R = o.fuse_modularity ? 6 : 2;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    HIPC(ctx, hipStreamBeginCapture(st(), hipStreamCaptureModeThreadLocal));
-    for (int j = 0; j < R; ++j) {
-      s = iteration(1 + j);
-      if (s) {
-        (void)hipStreamEndCapture(st(), &graph);
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipGetLastError();
-        return s;
+    int64_t done = 0;
+    while (true) {
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIPC(ctx, hipStreamBeginCapture(st(), hipStreamCaptureModeThreadLocal));
+      for (int j = 0; j < R; ++j) {
+        s = iteration(1 + j);
+        if (s) {
+          (void)hipStreamEndCapture(st(), &graph);
+          if (graph) (void)hipGraphDestroy(graph);
+          (void)hipGetLastError();
+          return s;
+        }
       }
-    }
-    HIPC(ctx, hipStreamEndCapture(st(), &graph));
-    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      (void)hipGraphDestroy(graph);
-      HIPC(ctx, e);
-    }
-    mgl_status rs = MGL_OK;
-    for (int rep = 0;; ++rep) {
-      if ((int64_t)rep * R > (1 << 22)) {
-        ctx->err = "inner loop did not terminate";
-        rs = MGL_DEVICE_ERROR;
-        break;
-      }
-      e = hipGraphLaunch(exec, st());
+      HIPC(ctx, hipStreamEndCapture(st(), &graph));
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
       if (e != hipSuccess) {
-        (void)hipGetLastError();
-        ctx->err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
-        rs = MGL_DEVICE_ERROR;
-        break;
+        (void)hipGraphDestroy(graph);
+        HIPC(ctx, e);
       }
-      int na;
-      rs = read_nactive(na);
-      if (rs || na == 0) break;
+      mgl_status rs = MGL_OK;
+      bool finished = false, recapture = false;
+      while (true) {
+        if (done * R > (1 << 22)) {
+          ctx->err = "inner loop did not terminate";
+          rs = MGL_DEVICE_ERROR;
+          break;
+        }
+        e = hipGraphLaunch(exec, st());
+        if (e != hipSuccess) {
+          (void)hipGetLastError();
+          ctx->err = std::string("hipGraphLaunch: ") + hipGetErrorString(e);
+          rs = MGL_DEVICE_ERROR;
+          break;
+        }
+        ++done;
+        int na;
+        rs = read_nactive(na);
+        if (rs) break;
+        if (na == 0) { finished = true; break; }
+        if (can_compact && 2 * na <= last_na) {
+          last_na = na;
+          recapture = true;
+          break;
+        }
+      }
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      if (rs) return rs;
+      if (finished) return MGL_OK;
+      if (recapture) {
+        s = compact();
+        if (s) return s;
+      }
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    return rs;
   }
 
   mgl_status level_end(int level);

@@ -52,8 +52,9 @@ struct MoveArgs {
 int launch_move_bin(int mode, int rec32, const BinSpec& bs, const MoveArgs& A, const vid* perm,
                     int64_t nb, hipStream_t st);
 // rec[v] = {c[v], a[c[v]]}; rec32 selects 8-byte records (unit mode, 2m < 2^31)
+// list (optional): the n vertex ids to process instead of 0..n-1
 void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
-                      hipStream_t st);
+                      hipStream_t st, const vid* list = nullptr);
 void move_kernels_allow_lds(int bytes);
 
 // ---- hub path (mgl_hub.hip) ------------------------------------------------
@@ -113,10 +114,11 @@ void launch_bin_scatter(const eid* off, int64_t n, const BinLayout& L,
 
 // ---- community-weight update, reductions, control --------------------------
 void launch_update(int mode, int64_t n, const vid* c, const vid* cn, const void* k, void* a,
-                   const int32_t* gid, GState* gs, hipStream_t st);
+                   const int32_t* gid, GState* gs, hipStream_t st, const vid* list = nullptr);
 // Adds sum(vin) (if vin) and sum(a^2) into each graph's accumulators.
+// with a list, its ids must be increasing (graphs stay contiguous)
 void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const int32_t* gid,
-                   GState* gs, hipStream_t st);
+                   GState* gs, hipStream_t st, const vid* list = nullptr);
 // Level start: active = run[g] and cleared accumulators; then, after the
 // singleton reduction, q_cur = Q(singletons).
 void launch_level_begin(int ng, GState* gs, const int32_t* run, hipStream_t st);
@@ -125,7 +127,12 @@ void launch_level_init(int ng, GState* gs, int exact, hipStream_t st);
 void launch_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
                    double* traj, int traj_max, int pre, hipStream_t st, int* nactive = nullptr);
 void launch_fixup(int64_t n, vid* c, vid* p, vid* nx, const int32_t* gid, const GState* gs,
-                  hipStream_t st);
+                  hipStream_t st, const vid* list = nullptr);
+// active-set compaction of a batch
+void launch_filter_bin(const vid* perm, int64_t nb, const int32_t* gid, const GState* gs,
+                       vid* out, unsigned long long* cnt, hipStream_t st);
+void launch_active_list(int64_t n, const eid* off, const int32_t* gid, const GState* gs,
+                        int32_t* flag, int64_t* pos, vid* out, void* scan_tmp, hipStream_t st);
 void launch_iota(vid* x, int64_t n, hipStream_t st);
 void launch_copy_k_to_a(int mode, int64_t n, const void* k, void* a, hipStream_t st);
 void launch_copy_vid(const vid* src, vid* dst, int64_t n, hipStream_t st);

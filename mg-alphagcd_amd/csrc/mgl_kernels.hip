@@ -65,9 +65,11 @@ __device__ __forceinline__ vid clamp_dst(vid dst, vid ci, int up, int version) {
 template <class KT, class AT>
 __global__ __launch_bounds__(256) void k_build_rec(int64_t n, const vid* __restrict__ c,
                                                    const KT* __restrict__ a,
-                                                   Rec<AT>* __restrict__ rec) {
-  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n;
-       v += (int64_t)gridDim.x * 256) {
+                                                   Rec<AT>* __restrict__ rec,
+                                                   const vid* __restrict__ list) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const int64_t v = list ? (int64_t)list[i] : i;
     const vid x = c[v];
     Rec<AT> r;
     r.c = x;
@@ -77,15 +79,18 @@ __global__ __launch_bounds__(256) void k_build_rec(int64_t n, const vid* __restr
 }
 
 void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
-                      hipStream_t st) {
+                      hipStream_t st, const vid* list) {
   if (n <= 0) return;
   const unsigned grid = grid_for(n, 256, 8192);
   if (mode == MODE_UNIT && rec32)
-    k_build_rec<int64_t, int32_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a, (Rec<int32_t>*)rec);
+    k_build_rec<int64_t, int32_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a,
+                                                        (Rec<int32_t>*)rec, list);
   else if (mode == MODE_UNIT || mode == MODE_INT)
-    k_build_rec<int64_t, int64_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a, (Rec<int64_t>*)rec);
+    k_build_rec<int64_t, int64_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a,
+                                                        (Rec<int64_t>*)rec, list);
   else
-    k_build_rec<double, double><<<grid, 256, 0, st>>>(n, c, (const double*)a, (Rec<double>*)rec);
+    k_build_rec<double, double><<<grid, 256, 0, st>>>(n, c, (const double*)a, (Rec<double>*)rec,
+                                                      list);
 }
 
 // Insert (key, w) into a table of 2^lg slots; the creator of a key also
@@ -968,11 +973,13 @@ template <class KT>
 __global__ __launch_bounds__(256) void k_update(int64_t n, const vid* __restrict__ c,
                                                 const vid* __restrict__ cn,
                                                 const KT* __restrict__ k, KT* a,
-                                                const int32_t* __restrict__ gid, GState* gs) {
+                                                const int32_t* __restrict__ gid, GState* gs,
+                                                const vid* __restrict__ list) {
   __shared__ unsigned int moved;
   if (threadIdx.x == 0) moved = 0;
   __syncthreads();
-  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t v = list ? (int64_t)list[i] : i;
     const vid x = c[v], y = cn[v];
     if (x != y) {
       const int g = gid ? gid[v] : 0;
@@ -989,13 +996,15 @@ __global__ __launch_bounds__(256) void k_update(int64_t n, const vid* __restrict
 }
 
 void launch_update(int mode, int64_t n, const vid* c, const vid* cn, const void* k, void* a,
-                   const int32_t* gid, GState* gs, hipStream_t st) {
+                   const int32_t* gid, GState* gs, hipStream_t st, const vid* list) {
   if (n <= 0) return;
   const unsigned grid = grid_for(n, 256, 8192);
   if (mode == MODE_UNIT || mode == MODE_INT)
-    k_update<int64_t><<<grid, 256, 0, st>>>(n, c, cn, (const int64_t*)k, (int64_t*)a, gid, gs);
+    k_update<int64_t><<<grid, 256, 0, st>>>(n, c, cn, (const int64_t*)k, (int64_t*)a, gid, gs,
+                                            list);
   else
-    k_update<double><<<grid, 256, 0, st>>>(n, c, cn, (const double*)k, (double*)a, gid, gs);
+    k_update<double><<<grid, 256, 0, st>>>(n, c, cn, (const double*)k, (double*)a, gid, gs,
+                                           list);
 }
 
 // ---------------------------------------------------------------------------
@@ -1056,20 +1065,22 @@ template <class KT, bool EXACT>
 __global__ __launch_bounds__(256) void k_reduce(int64_t n, const KT* __restrict__ vin,
                                                 const KT* __restrict__ a,
                                                 const int32_t* __restrict__ gid, GState* gs,
-                                                int64_t per_wave) {
+                                                int64_t per_wave, const vid* __restrict__ list) {
   const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   const int64_t beg = wave * per_wave;
   if (beg >= n) return;
   const int64_t end = beg + per_wave < n ? beg + per_wave : n;
   RedAcc<EXACT> acc;
-  int gcur = gid ? gid[beg] : 0;
+  auto vx = [&](int64_t i) -> int64_t { return list ? (int64_t)list[i] : i; };
+  int gcur = gid ? gid[vx(beg)] : 0;
   for (int64_t v0 = beg; v0 < end; v0 += 64) {
-    const int64_t v = v0 + lane;
-    const bool ok = v < end;
+    const int64_t i = v0 + lane;
+    const bool ok = i < end;
+    const int64_t v = ok ? vx(i) : 0;
     int g = gcur;
     if (gid) {
-      g = ok ? gid[v] : gid[end - 1];
+      g = ok ? gid[v] : gid[vx(end - 1)];
       const int gf = __shfl(g, 0);
       const int gl = __shfl(g, 63);
       if (gf != gcur || gl != gcur) {
@@ -1106,9 +1117,9 @@ __global__ __launch_bounds__(256) void k_reduce1(int64_t n, const KT* __restrict
 }
 
 void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const int32_t* gid,
-                   GState* gs, hipStream_t st) {
+                   GState* gs, hipStream_t st, const vid* list) {
   if (n <= 0) return;
-  if (!gid) {
+  if (!gid && !list) {
     const unsigned grid = grid_for(n, 256 * 8, 1024);
     if (mode == MODE_UNIT || mode == MODE_INT)
       k_reduce1<int64_t, true><<<grid, 256, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a, gs);
@@ -1121,10 +1132,10 @@ void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const in
   const unsigned grid = (unsigned)((waves + 3) / 4);
   if (mode == MODE_UNIT || mode == MODE_INT)
     k_reduce<int64_t, true><<<grid, 256, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a,
-                                                  gid, gs, per_wave);
+                                                  gid, gs, per_wave, list);
   else
     k_reduce<double, false><<<grid, 256, 0, st>>>(n, (const double*)vin, (const double*)a, gid,
-                                                  gs, per_wave);
+                                                  gs, per_wave, list);
 }
 
 // ---------------------------------------------------------------------------
@@ -1226,9 +1237,11 @@ void launch_decide(int ng, GState* gs, double thr, int max_iter, int fused, int 
 //   fix 3 (unfused, accepted)      : c := p
 __global__ __launch_bounds__(256) void k_fixup(int64_t n, vid* c, vid* p, vid* nx,
                                                const int32_t* __restrict__ gid,
-                                               const GState* __restrict__ gs) {
+                                               const GState* __restrict__ gs,
+                                               const vid* __restrict__ list) {
   if (!gid && gs[0].fix == 0) return;
-  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t v = list ? (int64_t)list[i] : i;
     const int g = gid ? gid[v] : 0;
     const int f = gs[g].fix;
     if (f == 0) continue;
@@ -1240,9 +1253,64 @@ __global__ __launch_bounds__(256) void k_fixup(int64_t n, vid* c, vid* p, vid* n
   }
 }
 void launch_fixup(int64_t n, vid* c, vid* p, vid* nx, const int32_t* gid, const GState* gs,
-                  hipStream_t st) {
+                  hipStream_t st, const vid* list) {
   if (n <= 0) return;
-  k_fixup<<<grid_for(n, 256, 8192), 256, 0, st>>>(n, c, p, nx, gid, gs);
+  k_fixup<<<grid_for(n, 256, 8192), 256, 0, st>>>(n, c, p, nx, gid, gs, list);
+}
+
+// Keeps, per bin, the vertices of graphs still active: perm_out[off[b] + j].
+__global__ __launch_bounds__(256) void k_filter_bin(const vid* __restrict__ perm, int64_t nb,
+                                                    const int32_t* __restrict__ gid,
+                                                    const GState* __restrict__ gs,
+                                                    vid* __restrict__ out,
+                                                    unsigned long long* __restrict__ cnt) {
+  __shared__ unsigned int lc;
+  __shared__ unsigned long long base;
+  for (int64_t i0 = (int64_t)blockIdx.x * 256; i0 < nb; i0 += (int64_t)gridDim.x * 256) {
+    if (threadIdx.x == 0) lc = 0;
+    __syncthreads();
+    const int64_t i = i0 + threadIdx.x;
+    vid v = 0;
+    bool keep = false;
+    unsigned int r = 0;
+    if (i < nb) {
+      v = perm[i];
+      keep = gs[gid ? gid[v] : 0].active != 0;
+      if (keep) r = atomicAdd(&lc, 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && lc) base = atomicAdd(cnt, (unsigned long long)lc);
+    __syncthreads();
+    if (keep) out[base + r] = v;
+    __syncthreads();
+  }
+}
+void launch_filter_bin(const vid* perm, int64_t nb, const int32_t* gid, const GState* gs,
+                       vid* out, unsigned long long* cnt, hipStream_t st) {
+  if (nb <= 0) return;
+  k_filter_bin<<<grid_for(nb, 256, 4096), 256, 0, st>>>(perm, nb, gid, gs, out, cnt);
+}
+
+// flag[v] = 1 for non-isolated vertices of active graphs
+__global__ void k_flag_active(int64_t n, const eid* __restrict__ off,
+                              const int32_t* __restrict__ gid, const GState* __restrict__ gs,
+                              int32_t* __restrict__ flag) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    flag[v] = (off[v + 1] > off[v] && gs[gid ? gid[v] : 0].active) ? 1 : 0;
+}
+__global__ void k_compact_ids(int64_t n, const int32_t* __restrict__ flag,
+                              const int64_t* __restrict__ pos, vid* __restrict__ out) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n;
+       v += (int64_t)gridDim.x * blockDim.x)
+    if (flag[v]) out[pos[v]] = (vid)v;
+}
+void launch_active_list(int64_t n, const eid* off, const int32_t* gid, const GState* gs,
+                        int32_t* flag, int64_t* pos, vid* out, void* scan_tmp, hipStream_t st) {
+  if (n <= 0) return;
+  k_flag_active<<<grid_for(n, 256, 8192), 256, 0, st>>>(n, off, gid, gs, flag);
+  scan_exclusive_i32(flag, pos, n, scan_tmp, st);
+  k_compact_ids<<<grid_for(n, 256, 8192), 256, 0, st>>>(n, flag, pos, out);
 }
 
 // ---------------------------------------------------------------------------

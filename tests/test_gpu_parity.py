@@ -212,3 +212,28 @@ def test_repeat_runs_identical(ctx, mgl):
     b = ctx.louvain(g, keep_levels=True)
     assert a.final_q == b.final_q
     np.testing.assert_array_equal(a.final_assign, b.final_assign)
+
+
+@pytest.mark.parametrize("graph", [True, False])
+@pytest.mark.parametrize("fused", [True, False])
+def test_batched_weighted_mixed_sizes(ctx, mgl, graph, fused):
+    # graphs of very different sizes finish at different iterations and levels,
+    # which exercises the batch compaction and per-graph coarsening
+    graphs = []
+    for i in range(24):
+        if i % 2 == 0:
+            graphs.append(mgl.gen_sbm(500 + 300 * i, 4 + i, min_size=8, deg_in=16, deg_out=3,
+                                      seed=200 + i, weights=1))
+        else:
+            graphs.append(mgl.gen_rmat(9 + (i % 5), 8 + i, seed=300 + i, weights=1,
+                                       n_limit=400 + 250 * i))
+    batch = ctx.louvain_batched(graphs, keep_levels=True, use_graph=graph,
+                                fuse_modularity=fused)
+    for g, b in zip(graphs, batch):
+        p = ctx.louvain(g, keep_levels=True, use_graph=False, fuse_modularity=fused)
+        assert len(b.levels) == len(p.levels)
+        for lb, lp in zip(b.levels, p.levels):
+            assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
+            np.testing.assert_array_equal(lb.qtraj, lp.qtraj)
+            np.testing.assert_array_equal(lb.assign, lp.assign)
+        np.testing.assert_array_equal(b.final_assign, p.final_assign)
