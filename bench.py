@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--full", action="store_true", help="also time the full multi-level run")
     ap.add_argument("--tuning-file", default=None)
     ap.add_argument("--stats-out", default=None, help="write per-pass statistics (JSON)")
+    ap.add_argument("--membench", type=int, default=0,
+                    help="also time a streaming copy of this many bytes (HBM calibration)")
     ap.add_argument("--autotune", default=None,
                     help="run the bin-layout autotuner on this graph, write the layout to "
                          "this path and time with it")
@@ -117,6 +119,10 @@ def main():
 
     ctx = mgl.Context(0)
     log(f"device LDS per workgroup: {ctx.lds_bytes()} bytes; bins: {ctx.bin_bounds()}")
+    copy_gbps = None
+    if args.membench:
+        copy_gbps = ctx.membench(args.membench, 5)
+        log(f"streaming copy of {args.membench} bytes: {copy_gbps:.1f} GB/s (read + write)")
     if args.autotune:
         t0 = time.perf_counter()
         ctx.autotune(g, args.autotune)
@@ -217,6 +223,8 @@ def main():
         "move_ms": move_ms,
         "cpu_baseline": cpu,
     }
+    if copy_gbps:
+        out["copy_kernel_GBps"] = copy_gbps
     if full:
         out["full_run"] = full
     print(json.dumps(out), flush=True)

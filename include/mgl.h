@@ -201,6 +201,9 @@ int mgl_version_major(void);
 int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins);
 /* LDS bytes one workgroup may use on the context's device (sizes the tables). */
 int mgl_device_lds_bytes(const mgl_ctx* ctx);
+/* Copies `bytes` between two device buffers `iters` times with a streaming
+ * copy kernel and returns the best read+write GB/s (HBM calibration). */
+mgl_status mgl_membench(mgl_ctx* ctx, int64_t bytes, int iters, double* gbps);
 /* Times the move pass of g under candidate bin layouts and writes the fastest
  * to path (JSON lines). */
 mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* path);

@@ -127,6 +127,7 @@ _sig("mgl_version_major", _i, [])
 _sig("mgl_bin_bounds", _i, [_vp, C.POINTER(_i64), _i])
 _sig("mgl_autotune", _i, [_vp, C.POINTER(_Csr), C.c_char_p])
 _sig("mgl_device_lds_bytes", _i, [_vp])
+_sig("mgl_membench", _i, [_vp, _i64, _i, C.POINTER(_d)])
 
 
 def _p(a, t):
@@ -332,6 +333,11 @@ class Context:
         buf = (C.c_int64 * 16)()
         nb = _lib.mgl_bin_bounds(self._h, buf, 16)
         return [buf[i] for i in range(nb)]
+
+    def membench(self, nbytes: int, iters: int = 5) -> float:
+        g = C.c_double()
+        self._err(_lib.mgl_membench(self._h, nbytes, iters, C.byref(g)))
+        return g.value
 
     def lds_bytes(self) -> int:
         return _lib.mgl_device_lds_bytes(self._h)

@@ -1463,6 +1463,42 @@ int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins) {
 
 int mgl_device_lds_bytes(const mgl_ctx* ctx) { return ctx ? ctx->lds_max : 0; }
 
+// Streaming copy of `bytes` between two device buffers with the library's
+// copy kernel (k_stream_copy), `iters` times; returns the best GB/s. Used to
+// calibrate HBM byte counters against a transfer of known size.
+__global__ __launch_bounds__(256) void k_stream_copy(const int4* __restrict__ s, int4* __restrict__ d,
+                                                     int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    d[i] = s[i];
+}
+mgl_status mgl_membench(mgl_ctx* ctx, int64_t bytes, int iters, double* gbps) {
+  if (!ctx) return MGL_INVALID_ARG;
+  if (bytes < 16 || iters < 1 || !gbps) FAIL(ctx, MGL_INVALID_ARG, "bad membench arguments");
+  HIPC(ctx, hipSetDevice(ctx->device));
+  DBuf a, b;
+  HIPC(ctx, a.ensure(bytes));
+  HIPC(ctx, b.ensure(bytes));
+  launch_fill_bytes0(a.p, bytes, ctx->st);
+  const int64_t n = bytes / 16;
+  hipEvent_t e0, e1;
+  HIPC(ctx, hipEventCreate(&e0));
+  HIPC(ctx, hipEventCreate(&e1));
+  double best = 0;
+  for (int i = 0; i < iters; ++i) {
+    HIPC(ctx, hipEventRecord(e0, ctx->st));
+    k_stream_copy<<<2048, 256, 0, ctx->st>>>(a.as<int4>(), b.as<int4>(), n);
+    HIPC(ctx, hipEventRecord(e1, ctx->st));
+    HIPC(ctx, hipEventSynchronize(e1));
+    float ms = 0;
+    HIPC(ctx, hipEventElapsedTime(&ms, e0, e1));
+    best = std::max(best, 2.0 * (double)(n * 16) / (ms * 1e-3) / 1e9);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *gbps = best;
+  return MGL_OK;
+}
+
 }  // extern "C"
 
 // ===========================================================================
