@@ -1923,6 +1923,24 @@ static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
       if (D.b[i].upper > top) L.b[L.nb++] = D.b[i];
     return L;
   };
+  // the top LDS bin split in two: 80 KB tables (two workgroups per CU) up to
+  // degree 8192, the largest table above
+  {
+    BinLayout L = D;
+    for (int i = 0; i < L.nb; ++i)
+      if (L.b[i].shape == SHAPE_BLOCK && L.b[i].upper > 8192) {
+        for (int j = L.nb; j > i; --j) L.b[j] = L.b[j - 1];
+        L.b[i] = {8192, SHAPE_BLOCK, 1024, 10944};
+        ++L.nb;
+        break;
+      }
+    out.push_back(L);
+    // and the same with 512-thread workgroups for the split part
+    BinLayout L2 = L;
+    for (int i = 0; i < L2.nb; ++i)
+      if (L2.b[i].upper == 8192 && L2.b[i].shape == SHAPE_BLOCK) L2.b[i].s = 512;
+    out.push_back(L2);
+  }
   // workgroups of 256 from degree 257
   out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_LANE, 1, 0}, {8, SHAPE_LANE, 1, 0},
                       {16, SHAPE_GROUP, 4, 5}, {32, SHAPE_GROUP, 8, 6},
