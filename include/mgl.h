@@ -113,6 +113,9 @@ const int32_t* mgl_result_level_assign(const mgl_result* r, int level); /* NULL 
 mgl_status mgl_result_level_graph(const mgl_result* r, int level, int64_t* n, int64_t* nnz,
                                   const int64_t** offsets, const int32_t** cols,
                                   const double** w);
+/* Wall time of the level's inner loop; *coarsen_seconds (optional) receives
+ * the time of the level's end (bookkeeping and coarsening). */
+double mgl_result_level_seconds(const mgl_result* r, int level, double* coarsen_seconds);
 const int32_t* mgl_result_final_assign(const mgl_result* r); /* dense labels of level-0 vertices */
 double mgl_result_final_q(const mgl_result* r);
 int mgl_result_precision(const mgl_result* r); /* precision actually used */

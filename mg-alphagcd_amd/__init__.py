@@ -101,6 +101,7 @@ _sig("mgl_result_level_graph", _i, [_vp, _i, C.POINTER(_i64), C.POINTER(_i64),
                                      C.POINTER(C.POINTER(_i64)), C.POINTER(C.POINTER(_i32)),
                                      C.POINTER(C.POINTER(_d))])
 _sig("mgl_result_final_assign", C.POINTER(_i32), [_vp])
+_sig("mgl_result_level_seconds", _d, [_vp, _i, C.POINTER(_d)])
 _sig("mgl_result_final_q", _d, [_vp])
 _sig("mgl_result_precision", _i, [_vp])
 _sig("mgl_result_seconds", _d, [_vp])
@@ -229,6 +230,8 @@ class Level:
     assign: Optional[np.ndarray] = None
     graph: Optional[CSR] = None  # input graph of this level (levels >= 1), weights as float64
     graph_w64: Optional[np.ndarray] = None
+    seconds: float = 0.0
+    coarsen_seconds: float = 0.0
 
 
 @dataclass
@@ -251,6 +254,9 @@ def _read_result(r) -> Result:
         ap = _lib.mgl_result_level_assign(r, l)
         assign = np.ctypeslib.as_array(ap, shape=(n,)).copy() if (ap and n > 0) else None
         lev = Level(n=n, q=_lib.mgl_result_level_q(r, l), iters=it, qtraj=qt, assign=assign)
+        cs = C.c_double()
+        lev.seconds = _lib.mgl_result_level_seconds(r, l, C.byref(cs))
+        lev.coarsen_seconds = cs.value
         gn, gm = C.c_int64(), C.c_int64()
         po, pc, pw = C.POINTER(_i64)(), C.POINTER(_i32)(), C.POINTER(_d)()
         if l >= 1 and _lib.mgl_result_level_graph(r, l, C.byref(gn), C.byref(gm), C.byref(po),

@@ -149,6 +149,8 @@ struct LevelOut {
   int64_t n = 0;
   double q = 0;
   int iters = 0;
+  double seconds = 0;        // inner loop of the level (wall clock)
+  double coarsen_seconds = 0;
   std::vector<double> qtraj;
   std::vector<int32_t> assign;
   int64_t gn = 0, gnnz = 0;
@@ -999,12 +1001,23 @@ mgl_status Engine::run(std::vector<mgl_result*>& out) {
     if (!any) break;
     for (int i = 0; i < G; ++i)
       if (running[i]) Q_old[i] = Q_host[i];
+    const auto tl0 = std::chrono::steady_clock::now();
     mgl_status s = level_start();
     if (s) return s;
     s = run_level();
     if (s) return s;
+    HIPC(ctx, hipStreamSynchronize(st()));
+    const auto tl1 = std::chrono::steady_clock::now();
+    std::vector<size_t> nlev(G);
+    for (int i = 0; i < G; ++i) nlev[i] = res[i]->levels.size();
     s = level_end(level);
     if (s) return s;
+    const auto tl2 = std::chrono::steady_clock::now();
+    for (int i = 0; i < G; ++i)
+      if (res[i]->levels.size() > nlev[i]) {
+        res[i]->levels.back().seconds = std::chrono::duration<double>(tl1 - tl0).count();
+        res[i]->levels.back().coarsen_seconds = std::chrono::duration<double>(tl2 - tl1).count();
+      }
     ++level;
   }
   // float modes: the reported final modularity is an FP64 reduction over the
@@ -1450,6 +1463,11 @@ const int32_t* mgl_result_final_assign(const mgl_result* r) {
   return r ? r->final_assign.data() : nullptr;
 }
 double mgl_result_final_q(const mgl_result* r) { return r ? r->final_q : NAN; }
+double mgl_result_level_seconds(const mgl_result* r, int l, double* coarsen_seconds) {
+  if (!r || l < 0 || l >= (int)r->levels.size()) return NAN;
+  if (coarsen_seconds) *coarsen_seconds = r->levels[l].coarsen_seconds;
+  return r->levels[l].seconds;
+}
 int mgl_result_precision(const mgl_result* r) { return r ? r->precision : -1; }
 double mgl_result_seconds(const mgl_result* r) { return r ? r->seconds : NAN; }
 void mgl_result_free(mgl_result* r) { delete r; }
