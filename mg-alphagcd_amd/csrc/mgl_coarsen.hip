@@ -40,6 +40,13 @@ __device__ __forceinline__ int64_t wave_incl_scan(int64_t x) {
   return x;
 }
 
+// value of lane src of the wavefront
+__device__ __forceinline__ int64_t bcast64(int64_t x, int src) {
+  const int lo = __shfl((int)(uint32_t)x, src, 64);
+  const int hi = __shfl((int)(uint32_t)((uint64_t)x >> 32), src, 64);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
 // block-wide exclusive scan of one value per thread; returns the block total
 __device__ __forceinline__ int64_t block_excl_scan(int64_t x, int64_t& total) {
   __shared__ int64_t ws[SCAN_T / 64];
@@ -253,30 +260,46 @@ __global__ void k_newc(int64_t n, const vid* __restrict__ c, const int32_t* __re
   }
 }
 
-// keys of every entry of a kept row, written at e + delta[g] (compaction of
-// the kept graphs' entry ranges). A wavefront walks 64 rows at a time.
+// Keys of every entry of a kept row, written at e + delta[g] (compaction of
+// the kept graphs' entry ranges). Entry-parallel: a wavefront takes 256
+// consecutive entries, finds the rows spanning them once, and each lane finds
+// its entry's row by binary search over that short range of offsets.
+__device__ __forceinline__ int64_t row_of(const eid* __restrict__ off, int64_t lo, int64_t hi,
+                                          int64_t e) {
+  // largest r in [lo, hi] with off[r] <= e
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 template <class M, class VT>
-__global__ __launch_bounds__(256) void k_keys(int64_t n, const eid* __restrict__ off,
+__global__ __launch_bounds__(256) void k_keys(int64_t n, int64_t nnz, const eid* __restrict__ off,
                                               const vid* __restrict__ col, const void* w,
                                               const vid* __restrict__ newc,
                                               const int32_t* __restrict__ gid,
                                               const int64_t* __restrict__ delta, int bits,
                                               uint64_t* __restrict__ keys, VT* __restrict__ vals) {
-  // one wavefront per row for long rows is handled by striding lanes over
-  // entries; rows are visited wave-by-wave
   const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const int64_t nw = ((int64_t)gridDim.x * 256) >> 6;
-  for (int64_t v = wave; v < n; v += nw) {
-    const vid r = newc[v];
-    if (r < 0) continue;
+  const int64_t e0 = wave * 256;
+  if (e0 >= nnz) return;
+  const int64_t e1 = (e0 + 256 < nnz) ? e0 + 256 : nnz;
+  int64_t r = 0;
+  if (lane < 2) r = row_of(off, 0, n - 1, lane == 0 ? e0 : e1 - 1);
+  const int64_t rlo = bcast64(r, 0), rhi = bcast64(r, 1);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int64_t e = e0 + lane + 64 * j;
+    if (e >= e1) continue;
+    const int64_t v = row_of(off, rlo, rhi, e);
+    const vid rr = newc[v];
+    if (rr < 0) continue;
     const int64_t dl = delta ? delta[gid ? gid[v] : 0] : 0;
-    const eid e0 = off[v], e1 = off[v + 1];
-    for (eid e = e0 + lane; e < e1; e += 64) {
-      const uint64_t cc = (uint64_t)(uint32_t)newc[col[e]];
-      keys[e + dl] = ((uint64_t)(uint32_t)r << bits) | cc;
-      if (M::HAS_W) vals[e + dl] = (VT)((const typename M::W*)w)[e];
-    }
+    const uint64_t cc = (uint64_t)(uint32_t)newc[col[e]];
+    keys[e + dl] = ((uint64_t)(uint32_t)rr << bits) | cc;
+    if (M::HAS_W) vals[e + dl] = (VT)((const typename M::W*)w)[e];
   }
 }
 
@@ -286,10 +309,30 @@ __global__ void k_heads(const uint64_t* __restrict__ k, int64_t n, int32_t* __re
     head[i] = (i == 0 || k[i] != k[i - 1]) ? 1 : 0;
 }
 
-// Coarse entries from the sorted keys. Equal keys are contiguous, so each
-// wavefront sums the runs it holds with a segmented scan and the last lane of
-// a run adds the run's sum: about one global atomic per distinct key (plus
-// one per wavefront boundary inside long runs) instead of one per entry.
+// off2[lo..hi] = val, lo..hi of each lane; short ranges by the lane itself,
+// long ones (runs of empty coarse rows) by the whole wavefront
+__device__ __forceinline__ void fill_rows(eid* __restrict__ off2, int64_t lo, int64_t hi,
+                                          int64_t val) {
+  const int lane = threadIdx.x & 63;
+  const int64_t len = hi - lo + 1;
+  if (len > 0 && len <= 16)
+    for (int64_t r = lo; r <= hi; ++r) off2[r] = val;
+  uint64_t big = __ballot(len > 16);
+  while (big) {
+    const int src = __ffsll((unsigned long long)big) - 1;
+    const int64_t blo = bcast64(lo, src), bhi = bcast64(hi, src), bv = bcast64(val, src);
+    for (int64_t r = blo + lane; r <= bhi; r += 64) off2[r] = bv;
+    big &= big - 1;
+  }
+}
+
+// Coarse entries from the sorted keys. Equal keys are contiguous: each
+// wavefront sums the runs it holds with a segmented scan; a run that starts
+// and ends inside the wavefront is stored by its last lane, a run crossing a
+// wavefront boundary is added atomically (one atomic per wavefront it
+// touches). Row offsets come from the heads that change row: the head writes
+// the offsets of its row and of the empty rows before it; the last entry
+// writes those of the empty rows after it.
 template <class T>
 __device__ __forceinline__ T shfl_up_t(T v, int d) {
   if constexpr (sizeof(T) == 8) {
@@ -312,7 +355,7 @@ __global__ __launch_bounds__(256) void k_emit(const uint64_t* __restrict__ k,
                                               const int32_t* __restrict__ head,
                                               const int64_t* __restrict__ uidx, int bits,
                                               vid* __restrict__ col2, OT* __restrict__ w2,
-                                              int64_t* __restrict__ rowcnt) {
+                                              eid* __restrict__ off2, int64_t C, int64_t U) {
   const uint64_t mask = (bits >= 64) ? ~0ull : ((1ull << bits) - 1ull);
   const int lane = threadIdx.x & 63;
   for (int64_t base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63); base < n;
@@ -322,10 +365,18 @@ __global__ __launch_bounds__(256) void k_emit(const uint64_t* __restrict__ k,
     const uint64_t key = ok ? k[i] : ~0ull;
     OT x = ok ? (HAS_V ? (OT)v[i] : (OT)1) : (OT)0;
     int64_t u = ok ? uidx[i + 1] - 1 : -1;
+    int64_t glo = 0, ghi = -1, tlo = 0, thi = -1;
     if (ok && head[i]) {
       col2[u] = (vid)(key & mask);
-      atomicAdd((unsigned long long*)&rowcnt[key >> bits], 1ull);
+      glo = (i == 0) ? 0 : (int64_t)(k[i - 1] >> bits) + 1;
+      ghi = (int64_t)(key >> bits);
     }
+    if (ok && i == n - 1) {
+      tlo = (int64_t)(key >> bits) + 1;
+      thi = C;
+    }
+    fill_rows(off2, glo, ghi, u);
+    fill_rows(off2, tlo, thi, U);
     // segmented inclusive scan over equal u within the wavefront
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -333,18 +384,29 @@ __global__ __launch_bounds__(256) void k_emit(const uint64_t* __restrict__ k,
       const int64_t uy = shfl_up_t(u, d);
       if (lane >= d && uy == u) x += y;
     }
-    int64_t u_next;
+    int64_t u_next, u_first;
     {
       int lo = __shfl_down((int)(uint32_t)(uint64_t)u, 1, 64);
       int hi = __shfl_down((int)(uint32_t)((uint64_t)u >> 32), 1, 64);
       u_next = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+      lo = __shfl((int)(uint32_t)(uint64_t)u, 0, 64);
+      hi = __shfl((int)(uint32_t)((uint64_t)u >> 32), 0, 64);
+      u_first = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
     }
-    const bool last = ok && (lane == 63 || u_next != u || base + lane + 1 >= n);
+    const bool at_end = (i + 1 >= n);
+    const bool last = ok && (lane == 63 || u_next != u || at_end);
     if (last) {
-      if constexpr (std::is_same_v<OT, int64_t>)
+      // a run wholly inside this wavefront is stored; a run crossing a
+      // wavefront boundary is accumulated atomically (w2 starts at zero)
+      const bool started = (u != u_first) || head[base];
+      const bool ended = at_end || (lane < 63 ? u_next != u : head[i + 1] != 0);
+      if (started && ended) {
+        w2[u] = x;
+      } else if constexpr (std::is_same_v<OT, int64_t>) {
         atomicAdd((unsigned long long*)&w2[u], (unsigned long long)x);
-      else
+      } else {
         atomicAdd(&w2[u], x);
+      }
     }
   }
 }
@@ -365,8 +427,16 @@ static int coarsen_t(int64_t n, const eid* off, const vid* col, const void* w, c
   int bits = 1;
   while ((1ll << bits) < C) ++bits;
   const int kbits = 2 * bits;
-  k_keys<M, VT><<<gridc(n * 64, 256, 1u << 20), 256, 0, st>>>(n, off, col, w, B.newc, gid, delta,
-                                                             bits, B.k0, (VT*)B.v0);
+  int64_t nnz = 0;
+  if (n > 0) {
+    if (hipMemcpyAsync(&nnz, off + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+  }
+  if (nnz > 0) {
+    const int64_t waves = (nnz + 255) / 256;
+    k_keys<M, VT><<<(unsigned)((waves + 3) / 4), 256, 0, st>>>(n, nnz, off, col, w, B.newc, gid,
+                                                               delta, bits, B.k0, (VT*)B.v0);
+  }
   VT* vs = nullptr;
   uint64_t* ks = B.k0;
   if (e_keep > 0) {
@@ -385,13 +455,13 @@ static int coarsen_t(int64_t n, const eid* off, const vid* col, const void* w, c
   if (hipStreamSynchronize(st) != hipSuccess) return -1;
   // exclusive scan gives uidx[i] = count before i; emit needs the inclusive
   // count at i, i.e. uidx[i+1]; out[e_keep] = total makes that valid for all i
-  launch_fill_bytes0(B.rowcnt, sizeof(int64_t) * (size_t)(C + 1), st);
   launch_fill_bytes0(w2, sizeof(OT) * (size_t)(U > 0 ? U : 1), st);
   if (e_keep > 0)
     k_emit<VT, OT, HAS_V><<<gridc(e_keep, 256, 16384), 256, 0, st>>>(ks, vs, e_keep, B.head,
                                                                    B.uidx, bits, col2, (OT*)w2,
-                                                                   B.rowcnt);
-  scan_exclusive_i64(B.rowcnt, off2, C, B.scan_tmp, st);
+                                                                   off2, C, U);
+  else
+    launch_fill_bytes0(off2, sizeof(eid) * (size_t)(C + 1), st);
   *C_out = C;
   *E2_out = U;
   return 0;
@@ -401,23 +471,18 @@ int coarsen_level(int mode_in, int mode_out, int64_t n, const eid* off, const vi
                   const void* w, const vid* c, const int32_t* gid, const int32_t* keep,
                   const int64_t* delta, int64_t e_keep, CoarsenBufs& B, eid* off2, vid* col2,
                   void* w2, int64_t* C_out, int64_t* E2_out, hipStream_t st) {
-  (void)mode_out;
+#define C_(M_, VT_, HV_, OT_)                                                                \
+  return coarsen_t<M_, VT_, HV_, OT_>(n, off, col, w, c, gid, keep, delta, e_keep, B, off2,  \
+                                      col2, w2, C_out, E2_out, st)
   switch (mode_in) {
     case MODE_UNIT:
-      return coarsen_t<ModeUnit, int64_t, false, int64_t>(n, off, col, w, c, gid, keep, delta,
-                                                          e_keep, B, off2, col2, w2, C_out,
-                                                          E2_out, st);
-    case MODE_INT:
-      return coarsen_t<ModeInt, int64_t, true, int64_t>(n, off, col, w, c, gid, keep, delta,
-                                                        e_keep, B, off2, col2, w2, C_out, E2_out,
-                                                        st);
-    case MODE_F64:
-      return coarsen_t<ModeF64, double, true, double>(n, off, col, w, c, gid, keep, delta,
-                                                      e_keep, B, off2, col2, w2, C_out, E2_out,
-                                                      st);
-    case MODE_F32:
-      return coarsen_t<ModeF32, float, true, float>(n, off, col, w, c, gid, keep, delta, e_keep,
-                                                    B, off2, col2, w2, C_out, E2_out, st);
+      if (mode_out == MODE_I32) C_(ModeUnit, int32_t, false, int32_t);
+      C_(ModeUnit, int64_t, false, int64_t);
+    case MODE_I32: C_(ModeI32, int32_t, true, int32_t);
+    case MODE_INT: C_(ModeInt, int64_t, true, int64_t);
+    case MODE_F64: C_(ModeF64, double, true, double);
+    case MODE_F32: C_(ModeF32, float, true, float);
   }
+#undef C_
   return -1;
 }

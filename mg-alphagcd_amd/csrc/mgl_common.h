@@ -16,7 +16,12 @@ typedef unsigned __int128 u128;
 //   Int  : integer or dyadic fixed-point weights (scaled by 2^S) in int64.
 //   F64  : double weights, slots and totals (the reference's weight_t).
 //   F32  : float weights and slots; double totals, 2m and Q.
-enum ModeId { MODE_UNIT = 0, MODE_INT = 1, MODE_F64 = 2, MODE_F32 = 3 };
+//   I32  : integer weights below 2^31 (coarse levels of a unit graph whose 2m
+//          is below 2^31): int32 weights and slots, int64 totals.
+enum ModeId { MODE_UNIT = 0, MODE_INT = 1, MODE_F64 = 2, MODE_F32 = 3, MODE_I32 = 4 };
+__host__ __device__ inline bool mode_is_int(int m) {
+  return m == MODE_UNIT || m == MODE_INT || m == MODE_I32;
+}
 
 struct ModeUnit {
   typedef int32_t W; typedef int32_t S; typedef int64_t K;
@@ -24,6 +29,10 @@ struct ModeUnit {
 };
 struct ModeInt {
   typedef int64_t W; typedef int64_t S; typedef int64_t K;
+  static constexpr bool HAS_W = true; static constexpr bool EXACT = true;
+};
+struct ModeI32 {
+  typedef int32_t W; typedef int32_t S; typedef int64_t K;
   static constexpr bool HAS_W = true; static constexpr bool EXACT = true;
 };
 struct ModeF64 {

@@ -86,24 +86,32 @@ static size_t wsize(int mode) {  // bytes of one stored edge weight
   switch (mode) {
     case MODE_UNIT: return 0;
     case MODE_INT: return 8;
+    case MODE_I32: return 4;
     case MODE_F64: return 8;
     case MODE_F32: return 4;
   }
   return 8;
 }
-static bool mode_exact(int mode) { return mode == MODE_UNIT || mode == MODE_INT; }
-static int coarse_mode(int mode) { return mode == MODE_UNIT ? MODE_INT : mode; }
+static bool mode_exact(int mode) { return mode_is_int(mode); }
+// Weights of a coarse level: exact unit/integer levels whose graphs all have
+// 2m < 2^31 (rec32) carry int32 weights, since every coarse entry, degree and
+// community total is at most 2m; otherwise int64. Float modes keep theirs.
+static int coarse_mode(int mode, bool rec32) {
+  if (mode == MODE_UNIT || mode == MODE_I32) return rec32 ? MODE_I32 : MODE_INT;
+  return mode;
+}
 static int slot_bytes(int mode) {  // hash-slot value (partial sum)
   switch (mode) {
     case MODE_UNIT: return 4;
     case MODE_INT: return 8;
+    case MODE_I32: return 4;
     case MODE_F64: return 8;
     case MODE_F32: return 4;
   }
   return 8;
 }
 static int total_bytes(int mode, bool rec32) {  // community total carried in a record
-  return (mode == MODE_UNIT && rec32) ? 4 : 8;
+  return ((mode == MODE_UNIT || mode == MODE_I32) && rec32) ? 4 : 8;
 }
 // LDS bytes of one table slot: key + partial sum + community total
 static int slot_total_bytes(int mode, bool rec32) {
@@ -220,7 +228,7 @@ struct Engine {
       if (e) (void)hipEventDestroy(e);
   }
   hipStream_t st() const { return ctx->st; }
-  bool rec32_now() { return rec32 && LG().mode == MODE_UNIT; }
+  bool rec32_now() { return rec32 && (LG().mode == MODE_UNIT || LG().mode == MODE_I32); }
   const vid* vlist() const { return nlist >= 0 ? blist.as<vid>() : nullptr; }
   int64_t vcount() { return nlist >= 0 ? nlist : LG().n; }
   // device word: graphs still active after the last decision
@@ -296,7 +304,7 @@ struct Engine {
   // Binning of the current level (BIN::bin_create, BIN.cu:468-499).
   mgl_status make_bins() {
     LevelGraph& g = LG();
-    L = layout_for(ctx, g.mode, rec32 && g.mode == MODE_UNIT);
+    L = layout_for(ctx, g.mode, rec32_now());
     const int nb = L.nb;
     bin_n.assign(nb, 0);
     bin_off.assign(nb + 1, 0);
@@ -820,6 +828,12 @@ static mgl_status copy_level_graph_to_host(mgl_ctx* ctx, LevelGraph& g, int64_t 
                              hipMemcpyDeviceToHost, ctx->st));
     if (g.mode == MODE_UNIT) {
       std::fill(lo.gw.begin(), lo.gw.end(), 1.0);
+    } else if (g.mode == MODE_I32) {
+      std::vector<int32_t> w(lo.gnnz);
+      HIPC(ctx, hipMemcpyAsync(w.data(), g.w.as<int32_t>() + eb, 4 * lo.gnnz,
+                               hipMemcpyDeviceToHost, ctx->st));
+      HIPC(ctx, hipStreamSynchronize(ctx->st));
+      for (int64_t i = 0; i < lo.gnnz; ++i) lo.gw[i] = std::ldexp((double)w[i], -scale);
     } else if (g.mode == MODE_INT) {
       std::vector<int64_t> w(lo.gnnz);
       HIPC(ctx, hipMemcpyAsync(w.data(), g.w.as<int64_t>() + eb, 8 * lo.gnnz,
@@ -931,7 +945,7 @@ mgl_status Engine::level_end(int level) {
   HIPC(ctx, hipStreamSynchronize(st()));
   const int nxt = (cur == 1) ? 2 : 1;
   LevelGraph& ng = lg[nxt];
-  ng.mode = coarse_mode(g.mode);
+  ng.mode = coarse_mode(g.mode, rec32);
   // upper bounds: C <= n, E2 <= e_keep
   HIPC(ctx, ng.off.ensure(8 * (g.n + 2)));
   HIPC(ctx, ng.col.ensure(4 * std::max<int64_t>(e_keep, 1)));
@@ -1824,7 +1838,7 @@ mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64
     s->coarsen_ready = true;
   }
   LevelGraph& g = E.LG();
-  const int out_mode = coarse_mode(g.mode);
+  const int out_mode = coarse_mode(g.mode, E.rec32);
   DBuf doff, dcol, dw;
   HIPC(ctx, doff.ensure(8 * (g.n + 2)));
   HIPC(ctx, dcol.ensure(4 * std::max<int64_t>(g.nnz, 1)));
@@ -1848,7 +1862,11 @@ mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64
   if (off2) HIPC(ctx, hipMemcpy(off2, doff.p, 8 * (C + 1), hipMemcpyDeviceToHost));
   if (col2 && E2) HIPC(ctx, hipMemcpy(col2, dcol.p, 4 * E2, hipMemcpyDeviceToHost));
   if (w2 && E2) {
-    if (out_mode == MODE_INT) {
+    if (out_mode == MODE_I32) {
+      std::vector<int32_t> t(E2);
+      HIPC(ctx, hipMemcpy(t.data(), dw.p, 4 * E2, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < E2; ++i) w2[i] = std::ldexp((double)t[i], -E.scale);
+    } else if (out_mode == MODE_INT) {
       std::vector<int64_t> t(E2);
       HIPC(ctx, hipMemcpy(t.data(), dw.p, 8 * E2, hipMemcpyDeviceToHost));
       for (int64_t i = 0; i < E2; ++i) w2[i] = std::ldexp((double)t[i], -E.scale);

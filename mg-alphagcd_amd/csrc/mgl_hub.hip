@@ -328,6 +328,9 @@ int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStre
       return rec32 ? hub_t<ModeUnit, int32_t>(A, H, H.lds_slots, st)
                    : hub_t<ModeUnit, int64_t>(A, H, H.lds_slots, st);
     case MODE_INT: return hub_t<ModeInt, int64_t>(A, H, H.lds_slots, st);
+    case MODE_I32:
+      return rec32 ? hub_t<ModeI32, int32_t>(A, H, H.lds_slots, st)
+                   : hub_t<ModeI32, int64_t>(A, H, H.lds_slots, st);
     case MODE_F64: return hub_t<ModeF64, double>(A, H, H.lds_slots, st);
     case MODE_F32: return hub_t<ModeF32, double>(A, H, H.lds_slots, st);
   }
@@ -339,7 +342,7 @@ void hub_allow_lds(int bytes) {
   (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_>,         \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   A_(ModeUnit, int32_t) A_(ModeUnit, int64_t) A_(ModeInt, int64_t) A_(ModeF64, double)
-  A_(ModeF32, double)
+  A_(ModeF32, double) A_(ModeI32, int32_t) A_(ModeI32, int64_t)
 #undef A_
 }
 

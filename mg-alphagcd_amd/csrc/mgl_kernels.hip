@@ -82,10 +82,10 @@ void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* 
                       hipStream_t st, const vid* list) {
   if (n <= 0) return;
   const unsigned grid = grid_for(n, 256, 8192);
-  if (mode == MODE_UNIT && rec32)
+  if ((mode == MODE_UNIT || mode == MODE_I32) && rec32)
     k_build_rec<int64_t, int32_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a,
                                                         (Rec<int32_t>*)rec, list);
-  else if (mode == MODE_UNIT || mode == MODE_INT)
+  else if (mode_is_int(mode))
     k_build_rec<int64_t, int64_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a,
                                                         (Rec<int64_t>*)rec, list);
   else
@@ -774,6 +774,9 @@ int launch_move_bin(int mode, int rec32, const BinSpec& bs, const MoveArgs& A, c
       return rec32 ? move_bin_t<ModeUnit, int32_t>(bs, A, perm, nb, st)
                    : move_bin_t<ModeUnit, int64_t>(bs, A, perm, nb, st);
     case MODE_INT: return move_bin_t<ModeInt, int64_t>(bs, A, perm, nb, st);
+    case MODE_I32:
+      return rec32 ? move_bin_t<ModeI32, int32_t>(bs, A, perm, nb, st)
+                   : move_bin_t<ModeI32, int64_t>(bs, A, perm, nb, st);
     case MODE_F64: return move_bin_t<ModeF64, double>(bs, A, perm, nb, st);
     case MODE_F32: return move_bin_t<ModeF32, double>(bs, A, perm, nb, st);
   }
@@ -793,7 +796,7 @@ void move_kernels_allow_lds(int bytes) {
 #define B_(M_, AT_) A_(M_, AT_, 256) A_(M_, AT_, 512) A_(M_, AT_, 1024) \
   P_(M_, AT_, 256) P_(M_, AT_, 512) P_(M_, AT_, 1024)
   B_(ModeUnit, int32_t) B_(ModeUnit, int64_t) B_(ModeInt, int64_t) B_(ModeF64, double)
-  B_(ModeF32, double)
+  B_(ModeF32, double) B_(ModeI32, int32_t) B_(ModeI32, int64_t)
 #undef B_
 #undef P_
 #undef O_
@@ -876,6 +879,7 @@ void launch_rows_bin(int mode, int op, const BinSpec& bs, const RowArgs& A, cons
   switch (mode) {
     case MODE_UNIT: R_(ModeUnit) break;
     case MODE_INT: R_(ModeInt) break;
+    case MODE_I32: R_(ModeI32) break;
     case MODE_F64: R_(ModeF64) break;
     case MODE_F32: R_(ModeF32) break;
   }
@@ -999,7 +1003,7 @@ void launch_update(int mode, int64_t n, const vid* c, const vid* cn, const void*
                    const int32_t* gid, GState* gs, hipStream_t st, const vid* list) {
   if (n <= 0) return;
   const unsigned grid = grid_for(n, 256, 8192);
-  if (mode == MODE_UNIT || mode == MODE_INT)
+  if (mode_is_int(mode))
     k_update<int64_t><<<grid, 256, 0, st>>>(n, c, cn, (const int64_t*)k, (int64_t*)a, gid, gs,
                                             list);
   else
@@ -1121,7 +1125,7 @@ void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const in
   if (n <= 0) return;
   if (!gid && !list) {
     const unsigned grid = grid_for(n, 256 * 8, 1024);
-    if (mode == MODE_UNIT || mode == MODE_INT)
+    if (mode_is_int(mode))
       k_reduce1<int64_t, true><<<grid, 256, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a, gs);
     else
       k_reduce1<double, false><<<grid, 256, 0, st>>>(n, (const double*)vin, (const double*)a, gs);
@@ -1130,7 +1134,7 @@ void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const in
   const int64_t per_wave = 256;
   const int64_t waves = (n + per_wave - 1) / per_wave;
   const unsigned grid = (unsigned)((waves + 3) / 4);
-  if (mode == MODE_UNIT || mode == MODE_INT)
+  if (mode_is_int(mode))
     k_reduce<int64_t, true><<<grid, 256, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a,
                                                   gid, gs, per_wave, list);
   else
@@ -1330,7 +1334,7 @@ __global__ void k_copy(const T* s, T* d, int64_t n) {
 }
 void launch_copy_k_to_a(int mode, int64_t n, const void* k, void* a, hipStream_t st) {
   if (n <= 0) return;
-  if (mode == MODE_UNIT || mode == MODE_INT)
+  if (mode_is_int(mode))
     k_copy<int64_t><<<grid_for(n, 256, 8192), 256, 0, st>>>((const int64_t*)k, (int64_t*)a, n);
   else
     k_copy<double><<<grid_for(n, 256, 8192), 256, 0, st>>>((const double*)k, (double*)a, n);
