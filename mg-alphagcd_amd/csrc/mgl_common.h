@@ -116,6 +116,27 @@ __device__ __forceinline__ uint32_t any_slot(vid key, uint32_t H) {
   return (uint32_t)(((uint64_t)x * H) >> 32);
 }
 
+// Probe step of double hashing, from a second hash of the key. Tables of 2^lg
+// slots take an odd step; tables of any other size H are prime-sized and take
+// a step in [1, H-1]. Either way the step is coprime with the table size, so a
+// probe sequence visits every slot once in H probes.
+__device__ __forceinline__ uint32_t key_mix2(vid key) {
+  uint32_t x = (uint32_t)key ^ 0x5bd1e995u;
+  x *= 0x27d4eb2du;
+  x ^= x >> 15;
+  x *= 0x165667b1u;
+  return x ^ (x >> 13);
+}
+__device__ __forceinline__ uint32_t probe_step(vid key, uint32_t H, bool pow2, int lg) {
+  const uint32_t x = key_mix2(key);
+  (void)lg;
+  return pow2 ? ((x & (H - 1u)) | 1u) : 1u + (uint32_t)(((uint64_t)x * (H - 1u)) >> 32);
+}
+__device__ __forceinline__ uint32_t probe_next(uint32_t h, uint32_t step, uint32_t H) {
+  h += step;
+  return h >= H ? h - H : h;
+}
+
 // Total order used by the argmax (louvain.cu:544-554): higher gain first; on
 // equal gain the smaller id on up passes and the larger id on down passes
 // (tie_small forces "smaller" always).

@@ -118,6 +118,17 @@ static int slot_total_bytes(int mode, bool rec32) {
   return 4 + slot_bytes(mode) + total_bytes(mode, rec32);
 }
 
+// Largest prime <= x (x >= 2): BLOCK tables are prime-sized so that every
+// double-hashing step is coprime with the table size.
+static int prime_at_most(int x) {
+  for (; x > 2; --x) {
+    bool p = (x % 2) != 0;
+    for (int d = 3; p && (int64_t)d * d <= x; d += 2) p = (x % d) != 0;
+    if (p) return x;
+  }
+  return 2;
+}
+
 // Bin layout for a mode, with LDS tables shrunk to what the device allows.
 // BLOCK bins of at most 16 entries per thread use the owner-based kernel
 // (key + partial sum per slot); longer ones also keep the total in the slot.
@@ -131,6 +142,7 @@ static BinLayout layout_for(const mgl_ctx* ctx, int mode, bool rec32) {
       const int64_t cap =
           (int64_t)(ctx->lds_max - 2048) / (owner ? sb : slot_total_bytes(mode, rec32));
       if (b.size > cap) b.size = (int)cap;
+      b.size = prime_at_most(b.size);
       // every distinct community of a row needs a slot
       if (b.upper > b.size) b.upper = b.size;
     }

@@ -94,34 +94,29 @@ void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* 
 }
 
 // Insert (key, w) into a table of 2^lg slots; the creator of a key also
-// stores its community total. Keys are written once (EMPTY -> key).
+// stores its community total. Keys are written once (EMPTY -> key) by a
+// compare-and-swap that also reads the slot; collisions probe by double
+// hashing.
 template <class ST, class AT>
-__device__ __forceinline__ void ins_pow2(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, int lg) {
-  const uint32_t mask = (1u << lg) - 1u;
-  uint32_t h = slot_hash(key, lg);
-  while (true) {
-    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == MGL_EMPTY) {
-      cur = atomicCAS(&K[h], MGL_EMPTY, key);
-      if (cur == MGL_EMPTY) Av[h] = ak;
-    }
+__device__ __forceinline__ void ins_tab(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, uint32_t H,
+                                        bool pow2, int lg) {
+  uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
+  const uint32_t st = probe_step(key, H, pow2, lg);
+  for (uint32_t i = 0; i < H; ++i) {
+    const vid cur = atomicCAS(&K[h], MGL_EMPTY, key);
+    if (cur == MGL_EMPTY) Av[h] = ak;
     if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
-    h = (h + 1u) & mask;
+    h = probe_next(h, st, H);
   }
 }
-// Same for a table of any size H (multiply-high slot).
+template <class ST, class AT>
+__device__ __forceinline__ void ins_pow2(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, int lg) {
+  ins_tab<ST, AT>(K, V, Av, key, w, ak, 1u << lg, true, lg);
+}
+// Same for a prime table size H (multiply-high slot).
 template <class ST, class AT>
 __device__ __forceinline__ void ins_any(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, uint32_t H) {
-  uint32_t h = any_slot(key, H);
-  while (true) {
-    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (cur == MGL_EMPTY) {
-      cur = atomicCAS(&K[h], MGL_EMPTY, key);
-      if (cur == MGL_EMPTY) Av[h] = ak;
-    }
-    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
-    h = (h + 1u == H) ? 0u : h + 1u;
-  }
+  ins_tab<ST, AT>(K, V, Av, key, w, ak, H, false, 0);
 }
 
 #define MOVE_UNROLL 4
@@ -336,55 +331,40 @@ template <class ST>
 __device__ __forceinline__ int own_packed(uint64_t* T, vid key, ST w, uint32_t H, bool pow2,
                                           int lg) {
   uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
-  const uint64_t kk = (uint64_t)(uint32_t)key << 32;
-  while (true) {
-    uint64_t cur = __hip_atomic_load(&T[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if ((uint32_t)(cur >> 32) == (uint32_t)MGL_EMPTY) {
-      uint32_t wb;
-      __builtin_memcpy(&wb, &w, 4);
-      const uint64_t want = kk | wb;
-      const uint64_t old = atomicCAS((unsigned long long*)&T[h],
-                                     (unsigned long long)(((uint64_t)(uint32_t)MGL_EMPTY << 32)),
-                                     (unsigned long long)want);
-      if (old == ((uint64_t)(uint32_t)MGL_EMPTY << 32)) return (int)h;
-      cur = old;
-    }
-    if ((uint32_t)(cur >> 32) == (uint32_t)key) {
+  const uint32_t st = probe_step(key, H, pow2, lg);
+  uint32_t wb;
+  __builtin_memcpy(&wb, &w, 4);
+  const uint64_t empty = (uint64_t)(uint32_t)MGL_EMPTY << 32;
+  const uint64_t want = ((uint64_t)(uint32_t)key << 32) | wb;
+  for (uint32_t i = 0; i < H; ++i) {
+    const uint64_t old = atomicCAS((unsigned long long*)&T[h], (unsigned long long)empty,
+                                   (unsigned long long)want);
+    if (old == empty) return (int)h;
+    if ((uint32_t)(old >> 32) == (uint32_t)key) {
       atom_add((ST*)&T[h], w);  // the low half
       return -1;
     }
-    h = pow2 ? ((h + 1u) & ((1u << lg) - 1u)) : ((h + 1u == H) ? 0u : h + 1u);
+    h = probe_next(h, st, H);
   }
+  return -1;
 }
 
+// Separate key / sum arrays (8-byte sums): the CAS on the key decides the
+// creator; the sum is added by everyone.
 template <class ST>
-__device__ __forceinline__ int own_pow2(vid* K, ST* V, vid key, ST w, int lg) {
-  const uint32_t mask = (1u << lg) - 1u;
-  uint32_t h = slot_hash(key, lg);
-  while (true) {
-    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    bool mine = false;
-    if (cur == MGL_EMPTY) {
-      cur = atomicCAS(&K[h], MGL_EMPTY, key);
-      mine = (cur == MGL_EMPTY);
+__device__ __forceinline__ int own_sep(vid* K, ST* V, vid key, ST w, uint32_t H, bool pow2,
+                                       int lg) {
+  uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
+  const uint32_t st = probe_step(key, H, pow2, lg);
+  for (uint32_t i = 0; i < H; ++i) {
+    const vid cur = atomicCAS(&K[h], MGL_EMPTY, key);
+    if (cur == MGL_EMPTY || cur == key) {
+      atom_add(&V[h], w);
+      return cur == MGL_EMPTY ? (int)h : -1;
     }
-    if (mine || cur == key) { atom_add(&V[h], w); return mine ? (int)h : -1; }
-    h = (h + 1u) & mask;
+    h = probe_next(h, st, H);
   }
-}
-template <class ST>
-__device__ __forceinline__ int own_any(vid* K, ST* V, vid key, ST w, uint32_t H) {
-  uint32_t h = any_slot(key, H);
-  while (true) {
-    vid cur = __hip_atomic_load(&K[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    bool mine = false;
-    if (cur == MGL_EMPTY) {
-      cur = atomicCAS(&K[h], MGL_EMPTY, key);
-      mine = (cur == MGL_EMPTY);
-    }
-    if (mine || cur == key) { atom_add(&V[h], w); return mine ? (int)h : -1; }
-    h = (h + 1u == H) ? 0u : h + 1u;
-  }
+  return -1;
 }
 
 // Owner-table views: packed (key, 4-byte sum) words, or separate key / sum arrays.
@@ -418,7 +398,7 @@ struct OwnTab<ST, false> {
   }
   __device__ void clear(uint32_t i) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
   __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg) {
-    return pow2 ? own_pow2<ST>(K, V, key, w, lg) : own_any<ST>(K, V, key, w, H);
+    return own_sep<ST>(K, V, key, w, H, pow2, lg);
   }
   __device__ vid key(int i) const { return K[i]; }
   __device__ ST val(int i) const { return V[i]; }
