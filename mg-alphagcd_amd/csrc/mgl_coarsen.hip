@@ -163,30 +163,65 @@ __global__ __launch_bounds__(RS_T) void k_rs_hist(const uint64_t* __restrict__ k
   hist[(int64_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
+// One tile per workgroup: every key gets its stable rank within its digit
+// (wavefront ballots per round of 256 keys, running counts over the rounds),
+// the tile is staged in LDS in digit order, and then written out so that
+// consecutive threads store consecutive positions of one digit's run.
 template <class VT, bool HAS_V>
 __global__ __launch_bounds__(RS_T) void k_rs_scatter(const uint64_t* __restrict__ kin,
                                                      const VT* __restrict__ vin, int64_t n,
                                                      int shift, int64_t nblk,
+                                                     const int64_t* __restrict__ hist,
                                                      const int64_t* __restrict__ offs,
                                                      uint64_t* __restrict__ kout,
                                                      VT* __restrict__ vout) {
+  __shared__ uint64_t sk[RS_TILE];
+  __shared__ VT sv[HAS_V ? RS_TILE : 1];
   __shared__ int64_t gbase[256];
+  __shared__ unsigned int tstart[256];
   __shared__ unsigned int run[256];
   __shared__ unsigned int wc[RS_T / 64][256];
-  gbase[threadIdx.x] = offs[(int64_t)threadIdx.x * nblk + blockIdx.x];
-  run[threadIdx.x] = 0;
+  __shared__ unsigned int wsum[RS_T / 64];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  {
+    // digit offsets inside the tile: exclusive scan of the tile's histogram
+    const unsigned c = (unsigned)hist[(int64_t)threadIdx.x * nblk + blockIdx.x];
+    unsigned x = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned y = __shfl_up(x, d, 64);
+      if (l >= d) x += y;
+    }
+    if (l == 63) wsum[w] = x;
+    gbase[threadIdx.x] = offs[(int64_t)threadIdx.x * nblk + blockIdx.x];
+    run[threadIdx.x] = 0;
+    __syncthreads();
+    unsigned pre = 0;
+#pragma unroll
+    for (int i = 0; i < RS_T / 64; ++i) pre += (i < w) ? wsum[i] : 0u;
+    tstart[threadIdx.x] = pre + x - c;
+  }
   const uint64_t lt = (l == 0) ? 0ull : (~0ull >> (64 - l));
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  const int tn = (int)((n - base < RS_TILE) ? n - base : RS_TILE);
+  // the tile's keys (and values) in registers, all loads in flight at once
+  uint64_t kr[RS_I];
+  VT vr[RS_I];
+#pragma unroll
+  for (int r = 0; r < RS_I; ++r) {
+    const int jt = r * RS_T + threadIdx.x;
+    kr[r] = jt < tn ? kin[base + jt] : 0;
+    if (HAS_V) vr[r] = jt < tn ? vin[base + jt] : VT{};
+  }
+#pragma unroll
   for (int r = 0; r < RS_I; ++r) {
 #pragma unroll
     for (int i = 0; i < RS_T / 64; ++i) wc[i][threadIdx.x] = 0;
     __syncthreads();
-    const int64_t j = base + (int64_t)r * RS_T + threadIdx.x;
-    const bool ok = j < n;
-    uint64_t key = ok ? kin[j] : 0;
-    VT val{};
-    if (HAS_V && ok) val = vin[j];
+    const int jt = r * RS_T + threadIdx.x;
+    const bool ok = jt < tn;
+    const uint64_t key = kr[r];
+    const VT val = HAS_V ? vr[r] : VT{};
     const unsigned d = (unsigned)((key >> shift) & 255u);
     // lanes of this wave holding the same digit
     uint64_t peers = __ballot(ok);
@@ -212,11 +247,18 @@ __global__ __launch_bounds__(RS_T) void k_rs_scatter(const uint64_t* __restrict_
     }
     __syncthreads();
     if (ok) {
-      const int64_t pos = gbase[d] + wc[w][d] + rank_w;
-      kout[pos] = key;
-      if (HAS_V) vout[pos] = val;
+      const unsigned p = tstart[d] + wc[w][d] + rank_w;
+      sk[p] = key;
+      if (HAS_V) sv[p] = val;
     }
-    __syncthreads();
+    __syncthreads();  // wc is read above and cleared by the next round
+  }
+  for (int p = threadIdx.x; p < tn; p += RS_T) {
+    const uint64_t key = sk[p];
+    const unsigned d = (unsigned)((key >> shift) & 255u);
+    const int64_t pos = gbase[d] + (p - (int)tstart[d]);
+    kout[pos] = key;
+    if (HAS_V) vout[pos] = sv[p];
   }
 }
 
@@ -232,7 +274,8 @@ static uint64_t* radix_sort(uint64_t* k0, uint64_t* k1, VT* v0, VT* v1, int64_t 
     k_rs_hist<<<(unsigned)nblk, RS_T, 0, st>>>(ki, n, shift, nblk, hist);
     int64_t* offs = hist + 256 * nblk + 1;
     scan_exclusive_i64(hist, offs, 256 * nblk, scan_tmp, st);
-    k_rs_scatter<VT, HAS_V><<<(unsigned)nblk, RS_T, 0, st>>>(ki, vi, n, shift, nblk, offs, ko, vo);
+    k_rs_scatter<VT, HAS_V><<<(unsigned)nblk, RS_T, 0, st>>>(ki, vi, n, shift, nblk, hist, offs,
+                                                                ko, vo);
     uint64_t* tk = ki; ki = ko; ko = tk;
     VT* tv = vi; vi = vo; vo = tv;
   }

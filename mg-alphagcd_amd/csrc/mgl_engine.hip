@@ -423,8 +423,12 @@ struct Engine {
       hub.gk = hgk.as<vid>();
       hub.gv = hgv.p;
       hub.scan_tmp = hscan.p;
-      hub.lds_slots =
-          (uint32_t)((ctx->lds_max - 2048) / slot_total_bytes(g.mode, rec32_now()) / 16 * 16);
+      // owner tables of the aggregate: (key, sum) slots in half of the
+      // workgroup LDS, so that two buckets share a CU; prime-sized
+      {
+        const int sb = slot_bytes(g.mode) == 4 ? 8 : 4 + slot_bytes(g.mode);
+        hub.lds_slots = (uint32_t)prime_at_most((int)((ctx->lds_max / 2 - 1024) / sb));
+      }
     }
     return MGL_OK;
   }
@@ -1971,23 +1975,14 @@ static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
       if (D.b[i].upper > top) L.b[L.nb++] = D.b[i];
     return L;
   };
-  // the top LDS bin split in two: 80 KB tables (two workgroups per CU) up to
-  // degree 8192, the largest table above
+  // the built-in layout without its half-LDS bin (the largest table from
+  // degree 4097)
   {
-    BinLayout L = D;
-    for (int i = 0; i < L.nb; ++i)
-      if (L.b[i].shape == SHAPE_BLOCK && L.b[i].upper > 8192) {
-        for (int j = L.nb; j > i; --j) L.b[j] = L.b[j - 1];
-        L.b[i] = {8192, SHAPE_BLOCK, 1024, 10944};
-        ++L.nb;
-        break;
-      }
-    out.push_back(L);
-    // and the same with 512-thread workgroups for the split part
-    BinLayout L2 = L;
-    for (int i = 0; i < L2.nb; ++i)
-      if (L2.b[i].upper == 8192 && L2.b[i].shape == SHAPE_BLOCK) L2.b[i].s = 512;
-    out.push_back(L2);
+    BinLayout L;
+    L.nb = 0;
+    for (int i = 0; i < D.nb; ++i)
+      if (!(D.b[i].shape == SHAPE_BLOCK && D.b[i].upper == 8192)) L.b[L.nb++] = D.b[i];
+    if (L.nb != D.nb) out.push_back(L);
   }
   // workgroups of 256 from degree 257
   out.push_back(with({{0, SHAPE_NONE, 0, 0}, {4, SHAPE_LANE, 1, 0}, {8, SHAPE_LANE, 1, 0},

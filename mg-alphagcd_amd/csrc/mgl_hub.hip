@@ -19,11 +19,13 @@
 #pragma clang fp contract(off)
 
 #include "mgl_kernels.h"
+#include "mgl_tables.h"
 
 #define HUB_CHUNK 4096
 #define HUB_T 256
 #define HUB_ITEMS (HUB_CHUNK / HUB_T)
 #define HUB_MAXP 1024
+#define HUB_AGG_IT 8
 
 __device__ __forceinline__ uint32_t hub_bucket(vid key, uint32_t P) {
   return (uint32_t)(((uint64_t)((uint32_t)key * 2654435761u) * P) >> 32);
@@ -178,14 +180,71 @@ __host__ __device__ inline size_t hub_tab_bytes(uint32_t H, size_t sv, size_t sa
   return ((4 * (size_t)H + 15) / 16) * 16 + ((sv * H + 15) / 16) * 16 + sa * H;
 }
 
-template <class M, class AT>
-__global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hlds) {
+// Prime table sizes for a bucket: the smallest listed prime at or above 3/2
+// of the bucket's entries (load <= 2/3), capped by the largest LDS table.
+__device__ __forceinline__ uint32_t hub_table_size(int64_t cnt, uint32_t Hmax) {
+  const uint32_t primes[] = {509, 1021, 1531, 2039, 3067, 4093, 6143, 8191, 10223, 12281, 16381,
+                             20479};
+  const uint64_t want = (uint64_t)cnt + (uint64_t)cnt / 2;
+#pragma unroll
+  for (int i = 0; i < 12; ++i)
+    if (primes[i] >= want && primes[i] <= Hmax) return primes[i];
+  return Hmax;
+}
+
+// The largest listed prime at or below `bound` (bound >= 509).
+__device__ __forceinline__ uint32_t hub_prime_below(uint32_t bound) {
+  const uint32_t primes[] = {509, 1021, 1531, 2039, 3067, 4093, 6143, 8191, 10223, 12281, 16381,
+                             20479};
+  uint32_t r = 509;
+#pragma unroll
+  for (int i = 0; i < 12; ++i)
+    if (primes[i] <= bound) r = primes[i];
+  return r;
+}
+
+// Packed (key, 4-byte sum) insert into a prime table of H slots: the slot if
+// the key was created, -1 if it existed, -2 if every slot holds another key.
+template <class ST>
+__device__ __forceinline__ int hub_ins_packed(uint64_t* T, vid key, ST w, uint32_t H) {
+  uint32_t h = any_slot(key, H);
+  const uint32_t st = probe_step(key, H, false, 0);
+  uint32_t wb;
+  __builtin_memcpy(&wb, &w, 4);
+  const uint64_t empty = (uint64_t)(uint32_t)MGL_EMPTY << 32;
+  const uint64_t want = ((uint64_t)(uint32_t)key << 32) | wb;
+  for (uint32_t i = 0; i < H; ++i) {
+    const uint64_t old = atomicCAS((unsigned long long*)&T[h], (unsigned long long)empty,
+                                   (unsigned long long)want);
+    if (old == empty) return (int)h;
+    if ((uint32_t)(old >> 32) == (uint32_t)key) {
+      atom_add((ST*)&T[h], w);
+      return -1;
+    }
+    h = probe_next(h, st, H);
+  }
+  return -2;
+}
+
+// One workgroup per bucket. Buckets of at most IT entries per thread (and at
+// most 7/8 of the largest LDS table) use an owner-based LDS table as the move
+// kernels do: the creator of a key evaluates it with the total it staged, so
+// the table is neither scanned nor sized beyond the bucket. Larger buckets
+// (many entries of few communities) first try an LDS table that is scanned at
+// the end (4-byte sums); if their distinct communities overfill it, or the
+// sums take 8 bytes, an HBM table of 2*cnt slots at 2*s0 is used.
+template <class M, class AT, int IT>
+__global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hmax) {
   typedef typename M::S ST;
   typedef typename M::K KT;
+  constexpr bool PACK = sizeof(ST) == 4;
+  typedef OwnTab<ST, PACK> TB;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double rg[16];
   __shared__ vid rd[16];
   __shared__ int rc[16];
+  __shared__ int ovf;
+  __shared__ unsigned int ncre;
   const int64_t b = blockIdx.x;
   const int64_t i = H.bk_hub[b];
   const vid v = H.hub_v[i];
@@ -205,56 +264,98 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
     const double Mh = A.gs[g].mass / 2.0;
     const double deici = (double)((const KT*)H.hub_eici)[i];
     const double dki = (double)ki, daci = (double)aci_k;
-    // an LDS table when the bucket fits at load <= 3/4, else an HBM table of
-    // 2*cnt slots at 2*s0 (tables of different buckets do not overlap)
-    const bool lds = (uint64_t)cnt * 4 <= (uint64_t)Hlds * 3;
-    const uint32_t Ht = lds ? Hlds : (uint32_t)(2 * cnt);
-    vid* K;
-    ST* V;
-    AT* Av;
-    if (lds) {
-      K = (vid*)smem;
-      V = (ST*)(smem + (((size_t)4 * Hlds + 15) / 16) * 16);
-      Av = (AT*)((unsigned char*)V + ((sizeof(ST) * (size_t)Hlds + 15) / 16) * 16);
-    } else {
-      K = H.gk + 2 * s0;
-      V = (ST*)H.gv + 2 * s0;
-      Av = (AT*)H.ga + 2 * s0;
-    }
-    for (uint32_t s = threadIdx.x; s < Ht; s += 1024) { K[s] = MGL_EMPTY; V[s] = (ST)0; }
-    if (!lds) __threadfence();
-    __syncthreads();
     const AT* ba = (const AT*)H.bkt_a;
-    for (int64_t j = s0 + threadIdx.x; j < s1; j += 1024) {
-      const vid key = H.bkt_c[j];
-      const ST wv = M::HAS_W ? ((const ST*)H.bkt_w)[j] : (ST)1;
-      if (lds) tab_insert<ST, AT, false>(K, V, Av, key, wv, ba[j], Ht);
-      else tab_insert<ST, AT, true>(K, V, Av, key, wv, ba[j], Ht);
-    }
-    if (!lds) __threadfence();
-    __syncthreads();
-    for (uint32_t s = threadIdx.x; s < Ht; s += 1024) {
-      vid cj;
-      ST ev;
-      AT ak = 0;
-      if (lds) {
-        cj = K[s];
-        ev = V[s];
-        ak = Av[s];
-      } else {
-        cj = __hip_atomic_load(&K[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cj != MGL_EMPTY) {
-          ev = __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          ak = __hip_atomic_load(&Av[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (cnt <= (int64_t)IT * 1024 && (uint64_t)cnt * 8 <= (uint64_t)Hmax * 7) {
+      const uint32_t Ht = hub_table_size(cnt, Hmax);
+      TB tb;
+      tb.init(smem, Ht);
+      for (uint32_t s = threadIdx.x; s < Ht; s += 1024) tb.clear(s);
+      __syncthreads();
+      int own[IT];
+      AT oa[IT];
+      vid key[IT];
+      ST wv[IT];
+#pragma unroll
+      for (int j = 0; j < IT; ++j) {
+        const int64_t x = s0 + threadIdx.x + (int64_t)j * 1024;
+        key[j] = x < s1 ? H.bkt_c[x] : MGL_EMPTY;
+        if (key[j] != MGL_EMPTY) {
+          oa[j] = ba[x];
+          wv[j] = M::HAS_W ? ((const ST*)H.bkt_w)[x] : (ST)1;
         }
       }
-      if (cj != MGL_EMPTY) {
+#pragma unroll
+      for (int j = 0; j < IT; ++j)
+        own[j] = key[j] != MGL_EMPTY ? tb.ins(key[j], wv[j], Ht, false, 0) : -1;
+      __syncthreads();
+      own_gains<M, AT, IT, TB>(tb, own, oa, deici, dki, daci, Mh, small, best, dst, occ);
+    } else {
+      bool done = false;
+      if constexpr (PACK) {
+        // LDS table of Hb packed slots followed by the creators' totals
+        const uint32_t Hb = hub_prime_below((uint32_t)(8ull * Hmax / (8 + sizeof(AT))));
+        uint64_t* T = (uint64_t*)smem;
+        AT* Avs = (AT*)(smem + 8 * (size_t)Hb);
+        const uint64_t empty = (uint64_t)(uint32_t)MGL_EMPTY << 32;
+        for (uint32_t s = threadIdx.x; s < Hb; s += 1024) T[s] = empty;
+        if (threadIdx.x == 0) {
+          ovf = 0;
+          ncre = 0;
+        }
+        __syncthreads();
+        const unsigned lim = Hb / 8 * 7;
+        for (int64_t j = s0 + threadIdx.x; j < s1; j += 1024) {
+          if (__hip_atomic_load(&ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
+          const ST w1 = M::HAS_W ? ((const ST*)H.bkt_w)[j] : (ST)1;
+          const int h = hub_ins_packed<ST>(T, H.bkt_c[j], w1, Hb);
+          if (h >= 0) {
+            Avs[h] = ba[j];
+            if (atomicAdd(&ncre, 1u) >= lim)
+              __hip_atomic_store(&ovf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          } else if (h == -2) {
+            __hip_atomic_store(&ovf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          }
+        }
+        __syncthreads();
+        if (!ovf) {
+          for (uint32_t s = threadIdx.x; s < Hb; s += 1024) {
+            const uint64_t t = T[s];
+            const vid cj = (vid)(uint32_t)(t >> 32);
+            if (cj == MGL_EMPTY) continue;
+            const uint32_t lo = (uint32_t)t;
+            ST ev;
+            __builtin_memcpy(&ev, &lo, 4);
+            ++occ;
+            const double wij = gain((double)ev, deici, dki, daci, (double)Avs[s], Mh);
+            if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+          }
+          done = true;
+        }
+        __syncthreads();  // the LDS table is not reused below, but ovf is read above
+      }
+      if (!done) {
+      const uint32_t Ht = (uint32_t)(2 * cnt);
+      vid* K = H.gk + 2 * s0;
+      ST* V = (ST*)H.gv + 2 * s0;
+      AT* Av = (AT*)H.ga + 2 * s0;
+      for (uint32_t s = threadIdx.x; s < Ht; s += 1024) { K[s] = MGL_EMPTY; V[s] = (ST)0; }
+      __threadfence();
+      __syncthreads();
+      for (int64_t j = s0 + threadIdx.x; j < s1; j += 1024) {
+        const ST w1 = M::HAS_W ? ((const ST*)H.bkt_w)[j] : (ST)1;
+        tab_insert<ST, AT, true>(K, V, Av, H.bkt_c[j], w1, ba[j], Ht);
+      }
+      __threadfence();
+      __syncthreads();
+      for (uint32_t s = threadIdx.x; s < Ht; s += 1024) {
+        const vid cj = __hip_atomic_load(&K[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cj == MGL_EMPTY) continue;
+        const ST ev = __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const AT ak = __hip_atomic_load(&Av[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ++occ;
-        double wij = (double)ev;
-        const double acj = (double)ak;
-        wij -= (deici - dki * (daci - acj) / (2. * Mh));  // louvain.cu:540-543
-        wij /= Mh;
+        const double wij = gain((double)ev, deici, dki, daci, (double)ak, Mh);
         if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+      }
       }
     }
   }
@@ -316,8 +417,9 @@ static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t
     k_hub_copy<<<(unsigned)gr, 256, 0, st>>>(H.boff, H.bcur, H.nbuckets);
   }
   k_hub_scatter<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
-  const size_t lds = hub_tab_bytes(Hlds, sizeof(ST), sizeof(AT));
-  k_hub_aggregate<M, AT><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+  const size_t lds = (sizeof(ST) == 4) ? 8 * (size_t)Hlds
+                                       : ((4 * (size_t)Hlds + 15) / 16) * 16 + sizeof(ST) * (size_t)Hlds;
+  k_hub_aggregate<M, AT, HUB_AGG_IT><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
   k_hub_final<M, AT><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
   return 0;
 }
@@ -339,7 +441,7 @@ int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStre
 
 void hub_allow_lds(int bytes) {
 #define A_(M_, AT_)                                                        \
-  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_>,         \
+  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_, HUB_AGG_IT>, \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   A_(ModeUnit, int32_t) A_(ModeUnit, int64_t) A_(ModeInt, int64_t) A_(ModeF64, double)
   A_(ModeF32, double) A_(ModeI32, int32_t) A_(ModeI32, int64_t)

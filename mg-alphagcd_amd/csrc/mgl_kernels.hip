@@ -17,6 +17,7 @@
 #pragma clang fp contract(off)
 
 #include "mgl_kernels.h"
+#include "mgl_tables.h"
 #include <climits>
 
 static unsigned grid_for(int64_t n, int per_block, unsigned cap) {
@@ -27,28 +28,10 @@ static unsigned grid_for(int64_t n, int per_block, unsigned cap) {
 }
 
 // ---------------------------------------------------------------------------
-// atomics on any slot type
-__device__ __forceinline__ void atom_add(int32_t* p, int32_t x) { atomicAdd(p, x); }
-__device__ __forceinline__ void atom_add(int64_t* p, int64_t x) {
-  atomicAdd((unsigned long long*)p, (unsigned long long)x);
-}
-__device__ __forceinline__ void atom_add(float* p, float x) { atomicAdd(p, x); }
-__device__ __forceinline__ void atom_add(double* p, double x) { atomicAdd(p, x); }
-
 template <class M>
 __device__ __forceinline__ bool vertex_active(const MoveArgs& A, vid v, int& g) {
   g = A.gid ? A.gid[v] : 0;
   return A.gs[g].active != 0;
-}
-
-// Per-vertex gain of moving into cj (louvain.cu:540-543, same operation order):
-//   wij = e_{v,cj}; wij -= (eici - ki*(aci - acj)/(2*M)); wij /= M;  M = mass/2
-__device__ __forceinline__ double gain(double e, double eici, double ki, double aci,
-                                       double acj, double Mh) {
-  double wij = e;
-  wij -= (eici - ki * (aci - acj) / (2. * Mh));
-  wij /= Mh;
-  return wij;
 }
 
 __device__ __forceinline__ vid clamp_dst(vid dst, vid ci, int up, int version) {
@@ -319,91 +302,6 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Owner-based variants (rows of at most IT entries per lane): each lane keeps
-// in registers the entries it gathered and the slots whose key it created.
-// The table holds key and partial sum only; after the barrier every creator
-// evaluates its own keys with the total it gathered itself, so neither a
-// table scan nor a second gather is needed.
-// Tables whose sums take 4 bytes keep (sum, key) in one 64-bit word per slot
-// (low half = sum, high half = key): a new key is one 64-bit CAS that also
-// stores its first weight; a known key is one 32-bit add on the low half.
-template <class ST>
-__device__ __forceinline__ int own_packed(uint64_t* T, vid key, ST w, uint32_t H, bool pow2,
-                                          int lg) {
-  uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
-  const uint32_t st = probe_step(key, H, pow2, lg);
-  uint32_t wb;
-  __builtin_memcpy(&wb, &w, 4);
-  const uint64_t empty = (uint64_t)(uint32_t)MGL_EMPTY << 32;
-  const uint64_t want = ((uint64_t)(uint32_t)key << 32) | wb;
-  for (uint32_t i = 0; i < H; ++i) {
-    const uint64_t old = atomicCAS((unsigned long long*)&T[h], (unsigned long long)empty,
-                                   (unsigned long long)want);
-    if (old == empty) return (int)h;
-    if ((uint32_t)(old >> 32) == (uint32_t)key) {
-      atom_add((ST*)&T[h], w);  // the low half
-      return -1;
-    }
-    h = probe_next(h, st, H);
-  }
-  return -1;
-}
-
-// Separate key / sum arrays (8-byte sums): the CAS on the key decides the
-// creator; the sum is added by everyone.
-template <class ST>
-__device__ __forceinline__ int own_sep(vid* K, ST* V, vid key, ST w, uint32_t H, bool pow2,
-                                       int lg) {
-  uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
-  const uint32_t st = probe_step(key, H, pow2, lg);
-  for (uint32_t i = 0; i < H; ++i) {
-    const vid cur = atomicCAS(&K[h], MGL_EMPTY, key);
-    if (cur == MGL_EMPTY || cur == key) {
-      atom_add(&V[h], w);
-      return cur == MGL_EMPTY ? (int)h : -1;
-    }
-    h = probe_next(h, st, H);
-  }
-  return -1;
-}
-
-// Owner-table views: packed (key, 4-byte sum) words, or separate key / sum arrays.
-template <class ST, bool PACK>
-struct OwnTab;
-template <class ST>
-struct OwnTab<ST, true> {
-  uint64_t* T;
-  static constexpr size_t slot = 8;
-  __device__ void init(unsigned char* base, uint32_t) { T = (uint64_t*)base; }
-  __device__ void clear(uint32_t i) { T[i] = (uint64_t)(uint32_t)MGL_EMPTY << 32; }
-  __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg) {
-    return own_packed<ST>(T, key, w, H, pow2, lg);
-  }
-  __device__ vid key(int i) const { return (vid)(uint32_t)(T[i] >> 32); }
-  __device__ ST val(int i) const {
-    const uint32_t lo = (uint32_t)T[i];
-    ST x;
-    __builtin_memcpy(&x, &lo, 4);
-    return x;
-  }
-};
-template <class ST>
-struct OwnTab<ST, false> {
-  vid* K;
-  ST* V;
-  static constexpr size_t slot = 4 + sizeof(ST);
-  __device__ void init(unsigned char* base, uint32_t H) {
-    K = (vid*)base;
-    V = (ST*)(base + ((4 * (size_t)H + 15) / 16) * 16);
-  }
-  __device__ void clear(uint32_t i) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
-  __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg) {
-    return own_sep<ST>(K, V, key, w, H, pow2, lg);
-  }
-  __device__ vid key(int i) const { return K[i]; }
-  __device__ ST val(int i) const { return V[i]; }
-};
-
 // Gathers and inserts the (at most IT per lane) entries of v's row.
 template <class M, class AT, int IT, class TB>
 __device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0, eid e1,
@@ -437,20 +335,6 @@ __device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0
     } else if (cj == ci) {
       if (u[j] != v) eici += wv; else selfw += wv;
     }
-  }
-}
-
-template <class M, class AT, int IT, class TB>
-__device__ __forceinline__ void own_gains(const TB& tb, const int (&own)[IT], const AT (&oa)[IT],
-                                          double deici, double dki, double daci, double Mh,
-                                          bool small, double& best, vid& dst, int& cnt) {
-#pragma unroll
-  for (int j = 0; j < IT; ++j) {
-    if (own[j] < 0) continue;
-    ++cnt;
-    const vid cj = tb.key(own[j]);
-    const double wij = gain((double)tb.val(own[j]), deici, dki, daci, (double)oa[j], Mh);
-    if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
   }
 }
 
@@ -673,6 +557,10 @@ BinLayout default_bins(int lds_bytes, int slot_bytes) {
   const int64_t hmax = (int64_t)(lds_bytes - 2048) / slot_bytes / 16 * 16;
   int64_t up = hmax * 3 / 4;
   if (up > 16 * 1024) up = 16 * 1024;
+  // below it, rows up to 8192 entries in tables of half the LDS (two
+  // workgroups of 512 threads per CU) when such a table holds them at <= 7/8
+  const int64_t half = (int64_t)(lds_bytes / 2 - 1024) / slot_bytes;
+  if (up > 8192 && half * 7 >= 8192 * 8) L.b[i++] = {8192, SHAPE_BLOCK, 512, (int)half};
   if (up > 4096) L.b[i++] = {up, SHAPE_BLOCK, 1024, (int)hmax};
   L.b[i++] = {INT64_MAX, SHAPE_GLOBAL, 0, 0};
   L.nb = i;
@@ -979,13 +867,67 @@ __global__ __launch_bounds__(256) void k_update(int64_t n, const vid* __restrict
   if (threadIdx.x == 0 && moved) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)moved);
 }
 
+// Integer form: the moves of a workgroup are first summed per community in an
+// LDS table (many vertices of a pass join the same few communities, and their
+// global atomics would serialise on one address); each touched community then
+// takes one global atomic per workgroup. Integer sums make the order
+// irrelevant. A community whose probe sequence meets UPD_PROBES occupied slots
+// goes straight to the global atomic.
+#define UPD_SLOTS 1021
+#define UPD_PROBES 8
+__device__ __forceinline__ void upd_add(vid* tk, int64_t* tv, int64_t* a, vid key, int64_t x) {
+  uint32_t h = any_slot(key, UPD_SLOTS);
+  const uint32_t st = probe_step(key, UPD_SLOTS, false, 0);
+  for (int t = 0; t < UPD_PROBES; ++t) {
+    const vid cur = atomicCAS(&tk[h], MGL_EMPTY, key);
+    if (cur == MGL_EMPTY || cur == key) {
+      atom_add(&tv[h], x);
+      return;
+    }
+    h = probe_next(h, st, UPD_SLOTS);
+  }
+  atom_add(&a[key], x);
+}
+
+__global__ __launch_bounds__(256) void k_update_int(int64_t n, const vid* __restrict__ c,
+                                                    const vid* __restrict__ cn,
+                                                    const int64_t* __restrict__ k, int64_t* a,
+                                                    const int32_t* __restrict__ gid, GState* gs,
+                                                    const vid* __restrict__ list) {
+  __shared__ vid tk[UPD_SLOTS];
+  __shared__ int64_t tv[UPD_SLOTS];
+  __shared__ unsigned int moved;
+  for (int s = threadIdx.x; s < UPD_SLOTS; s += 256) {
+    tk[s] = MGL_EMPTY;
+    tv[s] = 0;
+  }
+  if (threadIdx.x == 0) moved = 0;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+    const int64_t v = list ? (int64_t)list[i] : i;
+    const vid x = c[v], y = cn[v];
+    if (x != y) {
+      const int g = gid ? gid[v] : 0;
+      if (gs[g].active) {
+        const int64_t kv = k[v];
+        upd_add(tk, tv, a, x, -kv);
+        upd_add(tk, tv, a, y, kv);
+        atomicAdd(&moved, 1u);
+      }
+    }
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < UPD_SLOTS; s += 256)
+    if (tk[s] != MGL_EMPTY && tv[s] != 0) atom_add(&a[tk[s]], tv[s]);
+  if (threadIdx.x == 0 && moved) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)moved);
+}
+
 void launch_update(int mode, int64_t n, const vid* c, const vid* cn, const void* k, void* a,
                    const int32_t* gid, GState* gs, hipStream_t st, const vid* list) {
   if (n <= 0) return;
   const unsigned grid = grid_for(n, 256, 8192);
   if (mode_is_int(mode))
-    k_update<int64_t><<<grid, 256, 0, st>>>(n, c, cn, (const int64_t*)k, (int64_t*)a, gid, gs,
-                                            list);
+    k_update_int<<<grid, 256, 0, st>>>(n, c, cn, (const int64_t*)k, (int64_t*)a, gid, gs, list);
   else
     k_update<double><<<grid, 256, 0, st>>>(n, c, cn, (const double*)k, (double*)a, gid, gs,
                                            list);
