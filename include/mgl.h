@@ -74,7 +74,15 @@ typedef struct {
   int validate;        /* 1 (default): check offsets, column range and symmetry */
   const char* tuning_file; /* bin thresholds written by mgl_autotune, or NULL for built-in defaults */
   int use_graph;       /* 1 (default): the inner loop of a level is captured once as a hipGraph and replayed */
+  int record_layout;   /* MGL_LAYOUT_*: where the per-vertex records gathered by the move kernels live */
 } mgl_opts;
+
+/* Record layout of a level (results are identical in every layout). */
+enum {
+  MGL_LAYOUT_AUTO = 0,   /* degree-bin order for levels of more than 2^19 vertices, else vertex order */
+  MGL_LAYOUT_VERTEX = 1, /* record of v at position v */
+  MGL_LAYOUT_BINNED = 2  /* records in degree-bin order: hub records share cache lines */
+};
 
 typedef struct mgl_ctx mgl_ctx;
 typedef struct mgl_result mgl_result;

@@ -38,6 +38,7 @@ _STATUS = {0: "OK", 1: "INVALID_ARG", 2: "INVALID_GRAPH", 3: "OOM", 4: "DEVICE_E
            5: "UNSUPPORTED"}
 PREC_AUTO, PREC_EXACT, PREC_FP64, PREC_FP32_HASH = 0, 1, 2, 3
 TIE_REFERENCE, TIE_SMALLEST = 0, 1
+LAYOUT_AUTO, LAYOUT_VERTEX, LAYOUT_BINNED = 0, 1, 2
 
 
 class MglError(RuntimeError):
@@ -61,7 +62,7 @@ class _Opts(C.Structure):
                 ("max_iter", C.c_int), ("precision", C.c_int), ("tie_rule", C.c_int),
                 ("fuse_modularity", C.c_int), ("keep_levels", C.c_int),
                 ("keep_graphs", C.c_int), ("validate", C.c_int), ("tuning_file", C.c_char_p),
-                ("use_graph", C.c_int)]
+                ("use_graph", C.c_int), ("record_layout", C.c_int)]
 
 
 class PassStats(C.Structure):
@@ -281,7 +282,8 @@ def _read_result(r) -> Result:
 
 def make_opts(version=2, threshold=1e-6, max_phases=1000, max_iter=0, precision=PREC_AUTO,
               tie_rule=TIE_REFERENCE, fuse_modularity=True, keep_levels=False,
-              keep_graphs=False, validate=1, tuning_file=None, use_graph=True) -> _Opts:
+              keep_graphs=False, validate=1, tuning_file=None, use_graph=True,
+              record_layout=0) -> _Opts:
     o = _Opts()
     _lib.mgl_opts_default(C.byref(o))
     o.version, o.threshold, o.max_phases, o.max_iter = version, threshold, max_phases, max_iter
@@ -290,6 +292,7 @@ def make_opts(version=2, threshold=1e-6, max_phases=1000, max_iter=0, precision=
     o.validate = int(validate)
     o.tuning_file = tuning_file.encode() if tuning_file else None
     o.use_graph = int(use_graph)
+    o.record_layout = int(record_layout)
     return o
 
 

@@ -44,9 +44,16 @@ struct ModeF32 {
   static constexpr bool HAS_W = true; static constexpr bool EXACT = false;
 };
 
+// Column and weight arrays are streamed once per pass: non-temporal loads keep
+// them from evicting the randomly gathered per-vertex records from L2.
+__device__ __forceinline__ vid colload(const vid* col, eid e) {
+  return __builtin_nontemporal_load(col + e);
+}
+
 template <class M>
 __device__ __forceinline__ typename M::S wload(const void* w, eid e) {
-  if constexpr (M::HAS_W) return (typename M::S)((const typename M::W*)w)[e];
+  if constexpr (M::HAS_W)
+    return (typename M::S)__builtin_nontemporal_load((const typename M::W*)w + e);
   else return (typename M::S)1;
 }
 

@@ -189,6 +189,9 @@ struct mgl_result {
 };
 
 static const int TRAJ_MAX = 4096;
+// Levels with more vertices than this keep their records in degree-bin order
+// (an 8-byte record array of 2^19 vertices fills 4 MB, one XCD's L2).
+static const int64_t LAYOUT_AUTO_MIN_N = (int64_t)1 << 19;
 
 // ---------------------------------------------------------------------------
 // Engine: runs the multi-level loop over a union of G graphs.
@@ -205,6 +208,10 @@ struct Engine {
   // per-vertex state
   DBuf bc, bp, bn, bk, ba, bvin, bperm, blab, bfin, bgid0, brec, bperm2, blist, bcnt2;
   int64_t nlist = -1;  // >= 0: per-vertex passes walk blist (active vertices of a batch)
+  // record layout of the current level: rpos[v] = position of v's record,
+  // rinv = its inverse, colr = the level's columns as record positions
+  DBuf brpos, brinv, bcolr;
+  bool rlay = false;
   bool rec32 = false;  // 8-byte records (unit mode, every graph's 2m < 2^31)
   vid* c = nullptr; vid* p = nullptr; vid* nx = nullptr;
   // per-graph
@@ -320,6 +327,7 @@ struct Engine {
     const int nb = L.nb;
     bin_n.assign(nb, 0);
     bin_off.assign(nb + 1, 0);
+    rlay = false;
     if (g.n == 0) return MGL_OK;
     unsigned long long* cnt = bcounts.as<unsigned long long>();
     launch_fill_bytes0(cnt, 8 * MGL_MAX_BINS * 2, st());
@@ -337,6 +345,16 @@ struct Engine {
     HIPC(ctx, hipMemcpyAsync(cursor, cur0.data(), 8 * MGL_MAX_BINS, hipMemcpyHostToDevice, st()));
     HIPC(ctx, hipStreamSynchronize(st()));
     launch_bin_scatter(g.off.as<eid>(), g.n, L, cursor, bperm.as<vid>(), st());
+    // records in degree-bin order once the record array outgrows an L2 slice
+    rlay = o.record_layout == MGL_LAYOUT_BINNED ||
+           (o.record_layout == MGL_LAYOUT_AUTO && g.n > LAYOUT_AUTO_MIN_N);
+    if (rlay) {
+      HIPC(ctx, brpos.ensure(4 * (size_t)n0));
+      HIPC(ctx, brinv.ensure(4 * (size_t)n0));
+      HIPC(ctx, bcolr.ensure(4 * (size_t)std::max<int64_t>(nnz0, 1)));
+      launch_record_layout(bperm.as<vid>(), g.n, brpos.as<vid>(), brinv.as<vid>(), st());
+      launch_remap_cols(g.col.as<vid>(), g.nnz, brpos.as<vid>(), bcolr.as<vid>(), st());
+    }
     // hub plan of the last bin: chunks of each row, hash buckets per hub
     hub = HubPlan();
     const int last = nb - 1;
@@ -437,12 +455,13 @@ struct Engine {
     LevelGraph& g = LG();
     MoveArgs A;
     A.off = g.off.as<eid>();
-    A.col = g.col.as<vid>();
+    A.col = rlay ? bcolr.as<vid>() : g.col.as<vid>();
     A.w = g.w.p;
     A.k = bk.p;
     A.c = from;
     A.a = ba.p;
     A.rec = brec.p;
+    A.rpos = rlay ? brpos.as<vid>() : nullptr;
     A.c_new = to;
     A.vin = vin;
     A.gid = g.gid.as<int32_t>();
@@ -460,7 +479,8 @@ struct Engine {
     LevelGraph& g = LG();
     MoveArgs A = move_args(from, to, up, vin);
     const int r32 = rec32_now() ? 1 : 0;
-    launch_build_rec(g.mode, r32, vcount(), from, ba.p, brec.p, st(), vlist());
+    launch_build_rec(g.mode, r32, vcount(), from, ba.p, brec.p, st(), vlist(),
+                     rlay ? brpos.as<vid>() : nullptr, rlay ? brinv.as<vid>() : nullptr);
     int ne = 0;
     for (int b = L.nb - 1; b >= 1; --b) {
       if (bin_n[b] == 0) continue;
@@ -1366,6 +1386,7 @@ void mgl_opts_default(mgl_opts* o) {
   o->validate = 1;
   o->tuning_file = nullptr;
   o->use_graph = 1;
+  o->record_layout = MGL_LAYOUT_AUTO;
 }
 
 int mgl_version_major(void) { return 1; }
@@ -1417,6 +1438,8 @@ static mgl_status check_opts(mgl_ctx* ctx, const mgl_opts* o) {
   if (o->max_phases < 0) FAIL(ctx, MGL_INVALID_ARG, "max_phases must be >= 0");
   if (o->tie_rule != MGL_TIE_REFERENCE && o->tie_rule != MGL_TIE_SMALLEST)
     FAIL(ctx, MGL_INVALID_ARG, "unknown tie rule");
+  if (o->record_layout < MGL_LAYOUT_AUTO || o->record_layout > MGL_LAYOUT_BINNED)
+    FAIL(ctx, MGL_INVALID_ARG, "unknown record layout");
   return MGL_OK;
 }
 

@@ -94,11 +94,12 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
   for (uint32_t b = threadIdx.x; b < P; b += HUB_T) hist[b] = 0;
   __syncthreads();
   const Rec<AT>* R = (const Rec<AT>*)A.rec;
-  const vid ci = R[v].c;
+  const vid vp = A.rpos ? A.rpos[v] : v;
+  const vid ci = R[vp].c;
   const bool v1 = A.version == 1;
   KT eici = 0, selfw = 0;
   for (int64_t e = e0 + threadIdx.x; e < e1; e += HUB_T) {
-    const vid u = A.col[e];
+    const vid u = colload(A.col, e);
     const Rec<AT> r = R[u];
     const vid cj = r.c;
     const ST wv = wload<M>(A.w, e);
@@ -112,7 +113,7 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
     } else {
       H.tmp_c[pos] = MGL_EMPTY;
       if (cj == ci) {
-        if (u != v) eici += (KT)wv;
+        if (u != vp) eici += (KT)wv;
         else selfw += (KT)wv;
       }
     }

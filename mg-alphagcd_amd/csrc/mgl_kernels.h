@@ -31,12 +31,14 @@ BinLayout default_bins(int lds_bytes, int slot_bytes);
 // ---- move ------------------------------------------------------------------
 struct MoveArgs {
   const eid* off;
-  const vid* col;
+  const vid* col;     // columns as record positions (rpos[u] when rpos is set)
   const void* w;
   const void* k;      // K[n]
   const vid* c;       // assignment the moves start from
   const void* a;      // K[n] community totals of c
-  const void* rec;    // Rec<AT>[n]: {c[u], a[c[u]]} of the assignment the moves start from
+  const void* rec;    // Rec<AT>[n]: {c[u], a[c[u]]} of the assignment the moves start from,
+                      // stored at position rpos[u]
+  const vid* rpos;    // [n] record position of each vertex (nullable = identity)
   vid* c_new;         // proposal
   void* vin;          // K[n]: per-vertex weight to own community incl. self-loop (nullable)
   const int32_t* gid; // per-vertex graph id (nullable = graph 0)
@@ -51,10 +53,17 @@ struct MoveArgs {
 // vertices. Returns 0, or -1 when no kernel instance matches the bin's shape.
 int launch_move_bin(int mode, int rec32, const BinSpec& bs, const MoveArgs& A, const vid* perm,
                     int64_t nb, hipStream_t st);
-// rec[v] = {c[v], a[c[v]]}; rec32 selects 8-byte records (unit mode, 2m < 2^31)
+// rec[rpos[v]] = {c[v], a[c[v]]}; rec32 selects 8-byte records (unit mode, 2m < 2^31)
 // list (optional): the n vertex ids to process instead of 0..n-1
+// rpos / rinv (optional): record position of each vertex and its inverse; with
+// rinv and no list, position i is written from vertex rinv[i] (sequential writes)
 void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
-                      hipStream_t st, const vid* list = nullptr);
+                      hipStream_t st, const vid* list = nullptr, const vid* rpos = nullptr,
+                      const vid* rinv = nullptr);
+// Record layout of a level: rpos[perm[i]] = i and rinv[i] = perm[i] for i < n;
+// colr[e] = rpos[col[e]] for e < nnz.
+void launch_record_layout(const vid* perm, int64_t n, vid* rpos, vid* rinv, hipStream_t st);
+void launch_remap_cols(const vid* col, int64_t nnz, const vid* rpos, vid* colr, hipStream_t st);
 void move_kernels_allow_lds(int bytes);
 
 // ---- hub path (mgl_hub.hip) ------------------------------------------------

@@ -45,35 +45,83 @@ __device__ __forceinline__ vid clamp_dst(vid dst, vid ci, int up, int version) {
 // community's total. The slot creator stores the total in the table, so the
 // gain loop reads no global memory. AT = int32 in the unit mode while
 // 2m < 2^31 (an 8-byte record), else the mode's total type (16 bytes).
+// Records of a large level are stored in degree-bin order (rpos): the
+// frequently gathered records of high-degree vertices then share cache lines.
 template <class KT, class AT>
 __global__ __launch_bounds__(256) void k_build_rec(int64_t n, const vid* __restrict__ c,
                                                    const KT* __restrict__ a,
                                                    Rec<AT>* __restrict__ rec,
-                                                   const vid* __restrict__ list) {
+                                                   const vid* __restrict__ list,
+                                                   const vid* __restrict__ rpos,
+                                                   const vid* __restrict__ rinv) {
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * 256) {
-    const int64_t v = list ? (int64_t)list[i] : i;
+    int64_t v, at;
+    if (list) {
+      v = (int64_t)list[i];
+      at = rpos ? (int64_t)rpos[v] : v;
+    } else if (rinv) {
+      v = (int64_t)rinv[i];
+      at = i;
+    } else {
+      v = i;
+      at = i;
+    }
     const vid x = c[v];
     Rec<AT> r;
     r.c = x;
     r.a = (AT)a[x];
-    rec[v] = r;
+    rec[at] = r;
   }
 }
 
 void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
-                      hipStream_t st, const vid* list) {
+                      hipStream_t st, const vid* list, const vid* rpos, const vid* rinv) {
   if (n <= 0) return;
   const unsigned grid = grid_for(n, 256, 8192);
   if ((mode == MODE_UNIT || mode == MODE_I32) && rec32)
     k_build_rec<int64_t, int32_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a,
-                                                        (Rec<int32_t>*)rec, list);
+                                                        (Rec<int32_t>*)rec, list, rpos, rinv);
   else if (mode_is_int(mode))
     k_build_rec<int64_t, int64_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a,
-                                                        (Rec<int64_t>*)rec, list);
+                                                        (Rec<int64_t>*)rec, list, rpos, rinv);
   else
     k_build_rec<double, double><<<grid, 256, 0, st>>>(n, c, (const double*)a, (Rec<double>*)rec,
-                                                      list);
+                                                      list, rpos, rinv);
+}
+
+__global__ __launch_bounds__(256) void k_record_layout(const vid* __restrict__ perm, int64_t n,
+                                                       vid* __restrict__ rpos,
+                                                       vid* __restrict__ rinv) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * 256) {
+    const vid v = perm[i];
+    rinv[i] = v;
+    rpos[v] = (vid)i;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_remap_cols(const vid* __restrict__ col, int64_t nnz,
+                                                    const vid* __restrict__ rpos,
+                                                    vid* __restrict__ colr) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nnz;
+       e += (int64_t)gridDim.x * 256)
+    colr[e] = rpos[colload(col, e)];
+}
+
+void launch_record_layout(const vid* perm, int64_t n, vid* rpos, vid* rinv, hipStream_t st) {
+  if (n <= 0) return;
+  k_record_layout<<<grid_for(n, 256, 16384), 256, 0, st>>>(perm, n, rpos, rinv);
+}
+
+void launch_remap_cols(const vid* col, int64_t nnz, const vid* rpos, vid* colr, hipStream_t st) {
+  if (nnz <= 0) return;
+  k_remap_cols<<<grid_for(nnz, 256, 65536), 256, 0, st>>>(col, nnz, rpos, colr);
+}
+
+// Record position of v (its own record and the id its self-loop carries in A.col).
+__device__ __forceinline__ vid rec_pos(const MoveArgs& A, vid v) {
+  return A.rpos ? A.rpos[v] : v;
 }
 
 // Insert (key, w) into a table of 2^lg slots; the creator of a key also
@@ -108,7 +156,7 @@ __device__ __forceinline__ void ins_any(vid* K, ST* V, AT* Av, vid key, ST w, AT
 // entries per lane per step (their gathers are issued together), inserting
 // candidates and summing the own community's weight (eici, self-loop).
 template <class M, class AT, bool POW2>
-__device__ __forceinline__ void scan_row(const MoveArgs& A, vid v, vid ci, eid e0, eid e1,
+__device__ __forceinline__ void scan_row(const MoveArgs& A, vid vp, vid ci, eid e0, eid e1,
                                          int lane, int stride, vid* K, typename M::S* V, AT* Av,
                                          uint32_t tab, typename M::S& eici,
                                          typename M::S& selfw) {
@@ -121,7 +169,7 @@ __device__ __forceinline__ void scan_row(const MoveArgs& A, vid v, vid ci, eid e
 #pragma unroll
     for (int j = 0; j < MOVE_UNROLL; ++j) {
       const eid ej = e + (eid)j * stride;
-      u[j] = ej < e1 ? A.col[ej] : MGL_EMPTY;
+      u[j] = ej < e1 ? colload(A.col, ej) : MGL_EMPTY;
     }
 #pragma unroll
     for (int j = 0; j < MOVE_UNROLL; ++j)
@@ -136,7 +184,7 @@ __device__ __forceinline__ void scan_row(const MoveArgs& A, vid v, vid ci, eid e
         if (POW2) ins_pow2<ST, AT>(K, V, Av, cj, wv, r[j].a, (int)tab);
         else ins_any<ST, AT>(K, V, Av, cj, wv, r[j].a, tab);
       } else if (cj == ci) {
-        if (u[j] != v) eici += wv; else selfw += wv;
+        if (u[j] != vp) eici += wv; else selfw += wv;
       }
     }
   }
@@ -172,10 +220,11 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
   vid ci = 0;
   KT acik = 0;
   if (ok) {
-    const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
+    const vid vp = rec_pos(A, v);
+    const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
     ci = rv.c;
     acik = (KT)rv.a;
-    scan_row<M, AT, true>(A, v, ci, A.off[v], A.off[v + 1], lane, S, K, V, Av, LG, eici, selfw);
+    scan_row<M, AT, true>(A, vp, ci, A.off[v], A.off[v + 1], lane, S, K, V, Av, LG, eici, selfw);
   }
   __syncthreads();
   if (ok) {
@@ -267,11 +316,12 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   ST* V = (ST*)(smem + ((4 * (size_t)H + 15) / 16) * 16);
   AT* Av = (AT*)((unsigned char*)V + ((sizeof(ST) * (size_t)H + 15) / 16) * 16);
   for (uint32_t i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
-  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
+  const vid vp = rec_pos(A, v);
+  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
   const vid ci = rv.c;
   __syncthreads();
   ST eici = 0, selfw = 0;
-  scan_row<M, AT, false>(A, v, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, K, V, Av, H, eici,
+  scan_row<M, AT, false>(A, vp, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, K, V, Av, H, eici,
                          selfw);
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
@@ -304,7 +354,7 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
 // ---------------------------------------------------------------------------
 // Gathers and inserts the (at most IT per lane) entries of v's row.
 template <class M, class AT, int IT, class TB>
-__device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0, eid e1,
+__device__ __forceinline__ void own_row(const MoveArgs& A, vid vp, vid ci, eid e0, eid e1,
                                         int lane, int stride, TB& tb, uint32_t H, bool pow2,
                                         int lg, int (&own)[IT], AT (&oa)[IT],
                                         typename M::S& eici, typename M::S& selfw) {
@@ -316,7 +366,7 @@ __device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0
 #pragma unroll
   for (int j = 0; j < IT; ++j) {
     const eid e = e0 + lane + (eid)j * stride;
-    u[j] = e < e1 ? A.col[e] : MGL_EMPTY;
+    u[j] = e < e1 ? colload(A.col, e) : MGL_EMPTY;
   }
 #pragma unroll
   for (int j = 0; j < IT; ++j)
@@ -333,7 +383,7 @@ __device__ __forceinline__ void own_row(const MoveArgs& A, vid v, vid ci, eid e0
       own[j] = tb.ins(cj, wv, H, pow2, lg);
       oa[j] = r[j].a;
     } else if (cj == ci) {
-      if (u[j] != v) eici += wv; else selfw += wv;
+      if (u[j] != vp) eici += wv; else selfw += wv;
     }
   }
 }
@@ -369,10 +419,11 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
 #pragma unroll
   for (int j = 0; j < IT; ++j) own[j] = -1;
   if (ok) {
-    const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
+    const vid vp = rec_pos(A, v);
+    const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
     ci = rv.c;
     acik = (KT)rv.a;
-    own_row<M, AT, IT, TB>(A, v, ci, A.off[v], A.off[v + 1], lane, S, tb, H, true, LG, own, oa,
+    own_row<M, AT, IT, TB>(A, vp, ci, A.off[v], A.off[v + 1], lane, S, tb, H, true, LG, own, oa,
                            eici, selfw);
   }
   __syncthreads();
@@ -424,13 +475,14 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   TB tb;
   tb.init(smem, H);
   for (uint32_t i = threadIdx.x; i < H; i += BS) tb.clear(i);
-  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[v];
+  const vid vp = rec_pos(A, v);
+  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
   const vid ci = rv.c;
   __syncthreads();
   ST eici = 0, selfw = 0;
   int own[IT];
   AT oa[IT];
-  own_row<M, AT, IT, TB>(A, v, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, tb, H, false, 0, own,
+  own_row<M, AT, IT, TB>(A, vp, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, tb, H, false, 0, own,
                          oa, eici, selfw);
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
@@ -474,7 +526,8 @@ __global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __rest
   int cnt = 0;
   if (ok) {
     const Rec<AT>* R = (const Rec<AT>*)A.rec;
-    const Rec<AT> rv = R[v];
+    const vid vp = rec_pos(A, v);
+    const Rec<AT> rv = R[vp];
     const vid ci = rv.c;
     const eid e0 = A.off[v];
     const int d = (int)(A.off[v + 1] - e0);
@@ -482,7 +535,7 @@ __global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __rest
     Rec<AT> r[D];
     ST w[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) u[j] = j < d ? A.col[e0 + j] : MGL_EMPTY;
+    for (int j = 0; j < D; ++j) u[j] = j < d ? colload(A.col, e0 + j) : MGL_EMPTY;
 #pragma unroll
     for (int j = 0; j < D; ++j) {
       if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
@@ -498,7 +551,7 @@ __global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __rest
       const vid cj = r[j].c;
       cand[j] = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
       if (!cand[j] && cj == ci) {
-        if (u[j] != v) eici += w[j]; else selfw += w[j];
+        if (u[j] != vp) eici += w[j]; else selfw += w[j];
       }
     }
     // merge duplicates into their first occurrence
@@ -700,7 +753,7 @@ __global__ __launch_bounds__(256) void k_rows(RowArgs A, const vid* __restrict__
     const eid e0 = A.off[v], e1 = A.off[v + 1];
     const vid cv = (OP == OP_INTRA) ? A.c[v] : 0;
     for (eid e = e0 + lane; e < e1; e += (S >= 256 ? 256 : S)) {
-      const vid u = A.col[e];
+      const vid u = colload(A.col, e);
       const KT wv = (KT)wload<M>(A.w, e);
       if (OP == OP_WEIGHT) {
         s0 += wv;

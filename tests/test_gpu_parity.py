@@ -237,3 +237,69 @@ def test_batched_weighted_mixed_sizes(ctx, mgl, graph, fused):
             np.testing.assert_array_equal(lb.qtraj, lp.qtraj)
             np.testing.assert_array_equal(lb.assign, lp.assign)
         np.testing.assert_array_equal(b.final_assign, p.final_assign)
+
+
+# ---- records in degree-bin order (forced on graphs below the automatic size) ----
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_binned_record_layout_parity(ctx, mgl, orc, version):
+    g = mgl.gen_rmat(16, 16, seed=1)
+    res = ctx.louvain(g, version=version, record_layout=mgl.LAYOUT_BINNED, keep_levels=True,
+                      keep_graphs=True)
+    ref = orc.louvain(g.offsets, g.cols, None, mode=0, version=version)
+    compare(res, ref)
+
+
+def test_binned_layout_hub_weighted_selfloops(ctx, mgl, orc):
+    g = hub_graph(mgl)
+    print(f"hub graph n={g.n} nnz={g.nnz}", flush=True)
+    res = ctx.louvain(g, record_layout=mgl.LAYOUT_BINNED, keep_levels=True, keep_graphs=True)
+    compare(res, orc.louvain(g.offsets, g.cols, None))
+    g = mgl.gen_sbm(1 << 14, 64, min_size=64, seed=5, weights=1)
+    S = orc.scale_of(g.w)
+    res = ctx.louvain(g, precision=mgl.PREC_EXACT, record_layout=mgl.LAYOUT_BINNED,
+                      keep_levels=True, keep_graphs=True)
+    compare(res, orc.louvain(g.offsets, g.cols, g.w.astype(np.float64), mode=0), scale=S)
+    # self-loops: the own-community weight excludes the loop, found by record position
+    g = mgl.CSR.from_edges(5, [(0, 0), (0, 1), (1, 2), (2, 2), (3, 4), (4, 4)])
+    res = ctx.louvain(g, record_layout=mgl.LAYOUT_BINNED, keep_levels=True, keep_graphs=True)
+    compare(res, orc.louvain(g.offsets, g.cols, None))
+
+
+def test_binned_layout_move_stage(ctx, mgl, orc):
+    g = mgl.gen_rmat(14, 16, seed=7)
+    s = ctx.session(g, record_layout=mgl.LAYOUT_BINNED)
+    rng = np.random.default_rng(1)
+    k = orc.vertex_weights(g.offsets)
+    for trial in range(3):
+        c = rng.integers(0, g.n, g.n).astype(np.int32)
+        c = c[c]
+        s.set_assign(c)
+        a = orc.community_totals(k, c, g.n)
+        for up in (True, False):
+            np.testing.assert_array_equal(
+                s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
+        assert s.modularity() == orc.modularity(g.offsets, g.cols, None, c, a, g.nnz)
+    s.close()
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_binned_layout_batched(ctx, mgl, fused):
+    # a batch with compaction (graphs finish at different iterations)
+    graphs = []
+    for i in range(12):
+        if i % 2 == 0:
+            graphs.append(mgl.gen_sbm(600 + 400 * i, 4 + i, min_size=8, deg_in=16, deg_out=3,
+                                      seed=500 + i, weights=1))
+        else:
+            graphs.append(mgl.gen_rmat(9 + (i % 5), 8 + i, seed=600 + i, weights=1,
+                                       n_limit=500 + 300 * i))
+    batch = ctx.louvain_batched(graphs, keep_levels=True, fuse_modularity=fused,
+                                record_layout=mgl.LAYOUT_BINNED)
+    for g, b in zip(graphs, batch):
+        p = ctx.louvain(g, keep_levels=True, fuse_modularity=fused,
+                        record_layout=mgl.LAYOUT_VERTEX)
+        assert len(b.levels) == len(p.levels)
+        for lb, lp in zip(b.levels, p.levels):
+            assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
+            np.testing.assert_array_equal(lb.assign, lp.assign)
+        np.testing.assert_array_equal(b.final_assign, p.final_assign)
