@@ -139,13 +139,19 @@ typedef struct {
   double update_ms;    /* last pass: community-weight update */
   double modularity_ms;/* last pass: sum_in / sum a^2 reduction (+ E-pass when not fused) */
   double pass_ms;      /* last pass: whole inner-loop body incl. commit */
-  double dominant_ms;  /* last pass: duration of the longest move-kernel launch */
+  double dominant_ms;  /* last pass: duration of the longest single-kernel move launch */
   int dominant_bin;    /* bin id of that launch */
   int64_t dominant_vertices, dominant_entries; /* vertices / CSR entries that launch processed */
   int64_t moved;       /* vertices whose proposal differs from their community */
   int64_t distinct;    /* (vertex, neighbour community) pairs aggregated in the move */
   int64_t dominant_distinct; /* the same, for the dominant launch */
   double q;            /* committed modularity */
+  char dominant_kernel[96]; /* name of the dominant launch's kernel (as profilers print it) */
+  double hub_ms;       /* the hub path (rows beyond the LDS-table bins: several kernels), or 0 */
+  int64_t hub_vertices, hub_entries, hub_distinct;
+  /* LDS hash tables of the move pass: key inserts, compare-and-swap probes, and
+   * inserts that needed more than one probe (first-probe hit rate = 1 - multi/inserts) */
+  int64_t lds_inserts, lds_probes, lds_multi_probe;
 } mgl_pass_stats;
 
 mgl_status mgl_session_create(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* o,
@@ -157,6 +163,11 @@ mgl_status mgl_session_reset(mgl_session* s, double* q0);
 /* One inner-loop iteration of louvain.cu:1548-1660: move, weight update,
  * modularity, commit-or-reject. *accepted = 1 when committed. */
 mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted);
+/* Counts what the next mgl_session_pass's move aggregates, without timing it
+ * or changing the state it starts from: distinct (vertex, community) pairs per
+ * bin and the LDS hash-probe statistics, reported in that pass's statistics.
+ * No counter runs inside a timed pass (counting takes global atomics). */
+mgl_status mgl_session_count(mgl_session* s);
 mgl_status mgl_session_stats(const mgl_session* s, mgl_pass_stats* st);
 /* Stage entry points on the session's level-0 graph. */
 /* Replaces BIN::bin_create (src/bin/BIN.cu:468-499): returns the bin sizes. */

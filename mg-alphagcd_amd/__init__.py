@@ -72,10 +72,15 @@ class PassStats(C.Structure):
                 ("dominant_bin", C.c_int), ("dominant_vertices", C.c_int64),
                 ("dominant_entries", C.c_int64), ("moved", C.c_int64),
                 ("distinct", C.c_int64), ("dominant_distinct", C.c_int64),
-                ("q", C.c_double)]
+                ("q", C.c_double), ("dominant_kernel", C.c_char * 96), ("hub_ms", C.c_double),
+                ("hub_vertices", C.c_int64), ("hub_entries", C.c_int64),
+                ("hub_distinct", C.c_int64), ("lds_inserts", C.c_int64),
+                ("lds_probes", C.c_int64), ("lds_multi_probe", C.c_int64)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["dominant_kernel"] = self.dominant_kernel.decode()
+        return d
 
 
 def _sig(name, res, args):
@@ -112,6 +117,7 @@ _sig("mgl_session_destroy", None, [_vp])
 _sig("mgl_session_reset", _i, [_vp, C.POINTER(_d)])
 _sig("mgl_session_pass", _i, [_vp, C.POINTER(_d), C.POINTER(_i)])
 _sig("mgl_session_stats", _i, [_vp, C.POINTER(PassStats)])
+_sig("mgl_session_count", _i, [_vp])
 _sig("mgl_session_bins", _i, [_vp, C.POINTER(_i64), C.POINTER(_i)])
 _sig("mgl_session_set_assign", _i, [_vp, C.POINTER(_i32)])
 _sig("mgl_session_move", _i, [_vp, _i, C.POINTER(_i32)])
@@ -387,6 +393,11 @@ class Session:
         q, acc = C.c_double(), C.c_int()
         self.ctx._err(_lib.mgl_session_pass(self._h, C.byref(q), C.byref(acc)))
         return q.value, bool(acc.value)
+
+    def count(self):
+        """Counters of the next pass's move (distinct pairs, LDS probes), untimed,
+        state unchanged; they appear in stats() after that pass."""
+        self.ctx._err(_lib.mgl_session_count(self._h))
 
     def stats(self) -> dict:
         s = PassStats()

@@ -180,3 +180,30 @@ __device__ __forceinline__ void group_argmax(double& bg, vid& bd, bool prefer_sm
     if (better(og, od, bg, bd, prefer_small)) { bg = og; bd = od; }
   }
 }
+
+// Workgroup-wide helpers (BS threads, wave64).
+template <int BS, class T>
+__device__ __forceinline__ T block_sum(T v, T* red /* [BS/64] */) {
+  v = group_sum<64>(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  T s = 0;
+#pragma unroll
+  for (int i = 0; i < BS / 64; ++i) s += red[i];
+  return s;
+}
+
+template <int BS>
+__device__ __forceinline__ void block_argmax(double& bg, vid& bd, bool small, double* rg,
+                                             vid* rd) {
+  group_argmax<64>(bg, bd, small);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) { rg[w] = bg; rd[w] = bd; }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < BS / 64; ++i)
+    if (better(rg[i], rd[i], bg, bd, small)) { bg = rg[i]; bd = rd[i]; }
+}

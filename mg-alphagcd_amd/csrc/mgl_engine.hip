@@ -237,7 +237,8 @@ struct Engine {
   hipEvent_t ev[2 * MGL_MAX_BINS + 8];
   bool ev_ok = false;
   mgl_pass_stats stats{};
-  bool timing = false;
+  bool timing = false;    // HIP events around each bin and stage
+  bool counting = false;  // per-bin distinct pairs and LDS probe counters (global atomics)
 
   explicit Engine(mgl_ctx* c_) : ctx(c_) {
     for (auto& e : ev) e = nullptr;
@@ -278,7 +279,9 @@ struct Engine {
     HIPC(ctx, bdelta.ensure(8 * G));
     HIPC(ctx, bgather.ensure(8 * (G + 1) * 2));
     HIPC(ctx, btraj.ensure(sizeof(double) * (size_t)G * TRAJ_MAX));
-    HIPC(ctx, bcounts.ensure(8 * MGL_MAX_BINS * 4));
+    // [0, MAX) bin counts, [MAX, 2 MAX) scatter cursors, [2 MAX, 3 MAX) distinct
+    // pairs per bin, [3 MAX] active graphs, [4 MAX, 4 MAX + 3) LDS probe counters
+    HIPC(ctx, bcounts.ensure(8 * (MGL_MAX_BINS * 4 + 4)));
     c = bc.as<vid>(); p = bp.as<vid>(); nx = bn.as<vid>();
     if (!ev_ok) {
       for (auto& x : ev) HIPC(ctx, hipEventCreate(&x));
@@ -470,6 +473,7 @@ struct Engine {
     A.version = o.version;
     A.tie_small = o.tie_rule == MGL_TIE_SMALLEST;
     A.stats = nullptr;
+    A.probes = nullptr;
     return A;
   }
 
@@ -485,7 +489,10 @@ struct Engine {
     for (int b = L.nb - 1; b >= 1; --b) {
       if (bin_n[b] == 0) continue;
       // per-bin count of aggregated (vertex, neighbour community) pairs
-      if (timing) A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS + b;
+      if (counting) {
+        A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS + b;
+        A.probes = bcounts.as<unsigned long long>() + 4 * MGL_MAX_BINS;
+      }
       if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b], st()));
       const int rc = (L.b[b].shape == SHAPE_GLOBAL)
                          ? launch_hub(g.mode, r32, A, hub, st())
@@ -1581,6 +1588,9 @@ struct mgl_session {
   bool coarsen_ready = false;
   mgl_pass_stats st{};
   std::vector<int64_t> hoff;  // host copy of the level-0 offsets (statistics)
+  // counters of the move of pass count_it, from mgl_session_count
+  int count_it = -1;
+  unsigned long long cnt[MGL_MAX_BINS + 3] = {};
 };
 
 __global__ void k_reactivate(GState* gs) {
@@ -1592,6 +1602,7 @@ __global__ void k_reactivate(GState* gs) {
 __global__ void k_zero_stats(GState* gs, unsigned long long* dist) {
   if (threadIdx.x == 0) gs[0].moved = 0;
   dist[threadIdx.x] = 0;
+  if (threadIdx.x < 3) dist[2 * MGL_MAX_BINS + threadIdx.x] = 0;  // LDS probe counters
 }
 
 static mgl_status session_prime(mgl_session* s) {
@@ -1704,19 +1715,8 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   HIPC(ctx, hipEventElapsedTime(&ms, ev[B + 0], ev[B + 4]));
   st.pass_ms = ms;
   st.dominant_ms = -1;
-  std::vector<int64_t> deg_sum;
-  for (int b = E.L.nb - 1; b >= 1; --b) {
-    if (E.bin_n[b] == 0) continue;
-    HIPC(ctx, hipEventElapsedTime(&ms, ev[2 * b], ev[2 * b + 1]));
-    if (ms > st.dominant_ms) {
-      st.dominant_ms = ms;
-      st.dominant_bin = b;
-    }
-  }
-  if (st.dominant_ms >= 0) {
-    const int b = st.dominant_bin;
-    st.dominant_vertices = E.bin_n[b];
-    // CSR entries of the dominant bin's vertices
+  // CSR entries of a bin's vertices
+  auto bin_entries = [&](int b, int64_t& ents) -> mgl_status {
     std::vector<int32_t> perm(E.bin_n[b]);
     HIPC(ctx, hipMemcpy(perm.data(), E.bperm.as<vid>() + E.bin_off[b], 4 * E.bin_n[b],
                         hipMemcpyDeviceToHost));
@@ -1726,15 +1726,48 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
       hoff.resize(g.n + 1);
       HIPC(ctx, hipMemcpy(hoff.data(), g.off.p, 8 * (g.n + 1), hipMemcpyDeviceToHost));
     }
-    int64_t ents = 0;
+    ents = 0;
     for (int32_t v : perm) ents += hoff[v + 1] - hoff[v];
-    st.dominant_entries = ents;
+    return MGL_OK;
+  };
+  // counters of this pass's move, if mgl_session_count ran on the same state
+  unsigned long long dh[MGL_MAX_BINS + 3] = {};
+  if (s->count_it == s->it) memcpy(dh, s->cnt, sizeof(dh));
+  // the dominant launch among bins served by one kernel; the hub path apart
+  int hub_bin = -1;
+  for (int b = E.L.nb - 1; b >= 1; --b) {
+    if (E.bin_n[b] == 0) continue;
+    HIPC(ctx, hipEventElapsedTime(&ms, ev[2 * b], ev[2 * b + 1]));
+    if (E.L.b[b].shape == SHAPE_GLOBAL) {
+      st.hub_ms = ms;
+      hub_bin = b;
+      continue;
+    }
+    if (ms > st.dominant_ms) {
+      st.dominant_ms = ms;
+      st.dominant_bin = b;
+    }
   }
-  unsigned long long dh[MGL_MAX_BINS];
-  HIPC(ctx, hipMemcpy(dh, dist, 8 * MGL_MAX_BINS, hipMemcpyDeviceToHost));
+  if (st.dominant_ms >= 0) {
+    const int b = st.dominant_bin;
+    st.dominant_vertices = E.bin_n[b];
+    r = bin_entries(b, st.dominant_entries);
+    if (r) return r;
+    st.dominant_distinct = (int64_t)dh[b];
+    move_kernel_name(E.LG().mode, E.rec32_now() ? 1 : 0, E.L.b[b], st.dominant_kernel,
+                     (int)sizeof(st.dominant_kernel));
+  }
+  if (hub_bin >= 0) {
+    st.hub_vertices = E.bin_n[hub_bin];
+    r = bin_entries(hub_bin, st.hub_entries);
+    if (r) return r;
+    st.hub_distinct = (int64_t)dh[hub_bin];
+  }
   st.distinct = 0;
   for (int b = 0; b < MGL_MAX_BINS; ++b) st.distinct += (int64_t)dh[b];
-  if (st.dominant_ms >= 0) st.dominant_distinct = (int64_t)dh[st.dominant_bin];
+  st.lds_inserts = (int64_t)dh[MGL_MAX_BINS];
+  st.lds_probes = (int64_t)dh[MGL_MAX_BINS + 1];
+  st.lds_multi_probe = (int64_t)dh[MGL_MAX_BINS + 2];
   st.moved = gs.moved;
   st.q = gs.q_cur;
   // a rejected proposal ends the level; a session keeps passing, so it
@@ -1747,6 +1780,33 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
     if (r) return r;
     HIPC(ctx, hipStreamSynchronize(ctx->st));
   }
+  return MGL_OK;
+}
+
+mgl_status mgl_session_count(mgl_session* s) {
+  if (!s) return MGL_INVALID_ARG;
+  mgl_ctx* ctx = s->ctx;
+  HIPC(ctx, hipSetDevice(ctx->device));
+  Engine& E = *s->E;
+  // the move of the next pass: same input, direction and output buffer, so
+  // nothing the pass reads is changed
+  const int it = s->it + 1;
+  const bool fused = E.o.fuse_modularity != 0;
+  const int up = fused ? (((it + 1) & 1) ? 1 : 0) : ((it & 1) ? 1 : 0);
+  const vid* from = fused ? E.p : E.c;
+  vid* to = fused ? E.nx : E.p;
+  unsigned long long* dist = E.bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS;
+  launch_fill_bytes0(dist, 8 * MGL_MAX_BINS, ctx->st);
+  launch_fill_bytes0(dist + 2 * MGL_MAX_BINS, 8 * 3, ctx->st);
+  E.counting = true;
+  mgl_status r = E.move(from, to, up, nullptr, false);
+  E.counting = false;
+  if (r) return r;
+  HIPC(ctx, hipMemcpyAsync(s->cnt, dist, 8 * MGL_MAX_BINS, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(ctx, hipMemcpyAsync(s->cnt + MGL_MAX_BINS, dist + 2 * MGL_MAX_BINS, 8 * 3,
+                           hipMemcpyDeviceToHost, ctx->st));
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  s->count_it = it;
   return MGL_OK;
 }
 

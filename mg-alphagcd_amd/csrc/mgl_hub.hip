@@ -207,7 +207,8 @@ __device__ __forceinline__ uint32_t hub_prime_below(uint32_t bound) {
 // Packed (key, 4-byte sum) insert into a prime table of H slots: the slot if
 // the key was created, -1 if it existed, -2 if every slot holds another key.
 template <class ST>
-__device__ __forceinline__ int hub_ins_packed(uint64_t* T, vid key, ST w, uint32_t H) {
+__device__ __forceinline__ int hub_ins_packed(uint64_t* T, vid key, ST w, uint32_t H,
+                                              ProbeCount& pc) {
   uint32_t h = any_slot(key, H);
   const uint32_t st = probe_step(key, H, false, 0);
   uint32_t wb;
@@ -217,13 +218,15 @@ __device__ __forceinline__ int hub_ins_packed(uint64_t* T, vid key, ST w, uint32
   for (uint32_t i = 0; i < H; ++i) {
     const uint64_t old = atomicCAS((unsigned long long*)&T[h], (unsigned long long)empty,
                                    (unsigned long long)want);
-    if (old == empty) return (int)h;
+    if (old == empty) { pc.add(i + 1); return (int)h; }
     if ((uint32_t)(old >> 32) == (uint32_t)key) {
       atom_add((ST*)&T[h], w);
+      pc.add(i + 1);
       return -1;
     }
     h = probe_next(h, st, H);
   }
+  pc.add(H);
   return -2;
 }
 
@@ -234,7 +237,7 @@ __device__ __forceinline__ int hub_ins_packed(uint64_t* T, vid key, ST w, uint32
 // (many entries of few communities) first try an LDS table that is scanned at
 // the end (4-byte sums); if their distinct communities overfill it, or the
 // sums take 8 bytes, an HBM table of 2*cnt slots at 2*s0 is used.
-template <class M, class AT, int IT>
+template <class M, class AT, int IT, bool CNT>
 __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hmax) {
   typedef typename M::S ST;
   typedef typename M::K KT;
@@ -258,6 +261,7 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
   double best = 0.0;
   vid dst = ci;
   int occ = 0;
+  ProbeCount pc;
   if (cnt > 0) {
     const KT ki = ((const KT*)A.k)[v];
     const KT* a = (const KT*)A.a;
@@ -287,7 +291,7 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
       }
 #pragma unroll
       for (int j = 0; j < IT; ++j)
-        own[j] = key[j] != MGL_EMPTY ? tb.ins(key[j], wv[j], Ht, false, 0) : -1;
+        own[j] = key[j] != MGL_EMPTY ? tb.ins(key[j], wv[j], Ht, false, 0, pc) : -1;
       __syncthreads();
       own_gains<M, AT, IT, TB>(tb, own, oa, deici, dki, daci, Mh, small, best, dst, occ);
     } else {
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
         for (int64_t j = s0 + threadIdx.x; j < s1; j += 1024) {
           if (__hip_atomic_load(&ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
           const ST w1 = M::HAS_W ? ((const ST*)H.bkt_w)[j] : (ST)1;
-          const int h = hub_ins_packed<ST>(T, H.bkt_c[j], w1, Hb);
+          const int h = hub_ins_packed<ST>(T, H.bkt_c[j], w1, Hb, pc);
           if (h >= 0) {
             Avs[h] = ba[j];
             if (atomicAdd(&ncre, 1u) >= lim)
@@ -361,6 +365,7 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
     }
   }
   // workgroup argmax (cnt is uniform, every thread reaches this point)
+  if constexpr (CNT) flush_probes(A.probes, pc);
   group_argmax<64>(best, dst, small);
   occ = group_sum<64>(occ);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -374,7 +379,7 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
     }
     H.bgain[b] = best;
     H.bdst[b] = dst;
-    if (A.stats && tot) atomicAdd(A.stats, (unsigned long long)tot);
+    if (CNT && A.stats && tot) atomicAdd(A.stats, (unsigned long long)tot);
   }
 }
 
@@ -420,7 +425,10 @@ static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t
   k_hub_scatter<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
   const size_t lds = (sizeof(ST) == 4) ? 8 * (size_t)Hlds
                                        : ((4 * (size_t)Hlds + 15) / 16) * 16 + sizeof(ST) * (size_t)Hlds;
-  k_hub_aggregate<M, AT, HUB_AGG_IT><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+  if (A.stats || A.probes)
+    k_hub_aggregate<M, AT, HUB_AGG_IT, true><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+  else
+    k_hub_aggregate<M, AT, HUB_AGG_IT, false><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
   k_hub_final<M, AT><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
   return 0;
 }
@@ -441,8 +449,10 @@ int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStre
 }
 
 void hub_allow_lds(int bytes) {
-#define A_(M_, AT_)                                                        \
-  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_, HUB_AGG_IT>, \
+#define A_(M_, AT_)                                                                   \
+  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_, HUB_AGG_IT, false>,     \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);         \
+  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_, HUB_AGG_IT, true>,      \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   A_(ModeUnit, int32_t) A_(ModeUnit, int64_t) A_(ModeInt, int64_t) A_(ModeF64, double)
   A_(ModeF32, double) A_(ModeI32, int32_t) A_(ModeI32, int64_t)

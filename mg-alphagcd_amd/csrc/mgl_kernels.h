@@ -47,12 +47,28 @@ struct MoveArgs {
   int version;
   int tie_small;
   unsigned long long* stats; // [0] distinct pairs (nullable)
+  unsigned long long* probes; // [3] LDS hash inserts, probes, multi-probe inserts (nullable)
 };
 
 // Launch the move kernel of one GROUP or BLOCK bin. perm points at the bin's
 // vertices. Returns 0, or -1 when no kernel instance matches the bin's shape.
 int launch_move_bin(int mode, int rec32, const BinSpec& bs, const MoveArgs& A, const vid* perm,
                     int64_t nb, hipStream_t st);
+// Name of the kernel launch_move_bin would launch for a bin (as profilers
+// print it); cnt names the counting variant.
+int move_kernel_name(int mode, int rec32, const BinSpec& bs, char* buf, int len,
+                     bool cnt = false);
+// Per-mode instantiation units of the move kernels (mgl_move_*.hip): launch, or
+// with name != nullptr only name, the kernel of a bin; -1 for a mode they lack.
+#define MGL_MOVE_UNIT_DECL(tag)                                                          \
+  int move_bin_##tag(int mode, int rec32, bool cnt, const BinSpec& bs, const MoveArgs& A, \
+                     const vid* perm, int64_t nb, hipStream_t st, char* name, int len);   \
+  void move_allow_lds_##tag(int bytes);
+MGL_MOVE_UNIT_DECL(unit)
+MGL_MOVE_UNIT_DECL(i32)
+MGL_MOVE_UNIT_DECL(int)
+MGL_MOVE_UNIT_DECL(f32)
+#undef MGL_MOVE_UNIT_DECL
 // rec[rpos[v]] = {c[v], a[c[v]]}; rec32 selects 8-byte records (unit mode, 2m < 2^31)
 // list (optional): the n vertex ids to process instead of 0..n-1
 // rpos / rinv (optional): record position of each vertex and its inverse; with

@@ -19,6 +19,7 @@
 #include "mgl_kernels.h"
 #include "mgl_tables.h"
 #include <climits>
+#include <cstdio>
 
 static unsigned grid_for(int64_t n, int per_block, unsigned cap) {
   int64_t g = (n + per_block - 1) / per_block;
@@ -28,17 +29,6 @@ static unsigned grid_for(int64_t n, int per_block, unsigned cap) {
 }
 
 // ---------------------------------------------------------------------------
-template <class M>
-__device__ __forceinline__ bool vertex_active(const MoveArgs& A, vid v, int& g) {
-  g = A.gid ? A.gid[v] : 0;
-  return A.gs[g].active != 0;
-}
-
-__device__ __forceinline__ vid clamp_dst(vid dst, vid ci, int up, int version) {
-  if (version == 1) return dst;                      // louvain_gl.cu:578
-  return up ? (dst < ci ? dst : ci) : (dst > ci ? dst : ci);  // louvain.cu:579-585
-}
-
 // ---------------------------------------------------------------------------
 // Per-vertex record {c[u], a[c[u]]}, rebuilt before every move pass, so that
 // one gather per CSR entry brings both the neighbour's community and that
@@ -119,477 +109,6 @@ void launch_remap_cols(const vid* col, int64_t nnz, const vid* rpos, vid* colr, 
   k_remap_cols<<<grid_for(nnz, 256, 65536), 256, 0, st>>>(col, nnz, rpos, colr);
 }
 
-// Record position of v (its own record and the id its self-loop carries in A.col).
-__device__ __forceinline__ vid rec_pos(const MoveArgs& A, vid v) {
-  return A.rpos ? A.rpos[v] : v;
-}
-
-// Insert (key, w) into a table of 2^lg slots; the creator of a key also
-// stores its community total. Keys are written once (EMPTY -> key) by a
-// compare-and-swap that also reads the slot; collisions probe by double
-// hashing.
-template <class ST, class AT>
-__device__ __forceinline__ void ins_tab(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, uint32_t H,
-                                        bool pow2, int lg) {
-  uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
-  const uint32_t st = probe_step(key, H, pow2, lg);
-  for (uint32_t i = 0; i < H; ++i) {
-    const vid cur = atomicCAS(&K[h], MGL_EMPTY, key);
-    if (cur == MGL_EMPTY) Av[h] = ak;
-    if (cur == MGL_EMPTY || cur == key) { atom_add(&V[h], w); return; }
-    h = probe_next(h, st, H);
-  }
-}
-template <class ST, class AT>
-__device__ __forceinline__ void ins_pow2(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, int lg) {
-  ins_tab<ST, AT>(K, V, Av, key, w, ak, 1u << lg, true, lg);
-}
-// Same for a prime table size H (multiply-high slot).
-template <class ST, class AT>
-__device__ __forceinline__ void ins_any(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, uint32_t H) {
-  ins_tab<ST, AT>(K, V, Av, key, w, ak, H, false, 0);
-}
-
-#define MOVE_UNROLL 4
-
-// Walks the row of v with `stride` lanes starting at `lane`, MOVE_UNROLL
-// entries per lane per step (their gathers are issued together), inserting
-// candidates and summing the own community's weight (eici, self-loop).
-template <class M, class AT, bool POW2>
-__device__ __forceinline__ void scan_row(const MoveArgs& A, vid vp, vid ci, eid e0, eid e1,
-                                         int lane, int stride, vid* K, typename M::S* V, AT* Av,
-                                         uint32_t tab, typename M::S& eici,
-                                         typename M::S& selfw) {
-  typedef typename M::S ST;
-  const Rec<AT>* R = (const Rec<AT>*)A.rec;
-  const bool v1 = A.version == 1;
-  for (eid e = e0 + lane; e < e1; e += (eid)stride * MOVE_UNROLL) {
-    vid u[MOVE_UNROLL];
-    Rec<AT> r[MOVE_UNROLL];
-#pragma unroll
-    for (int j = 0; j < MOVE_UNROLL; ++j) {
-      const eid ej = e + (eid)j * stride;
-      u[j] = ej < e1 ? colload(A.col, ej) : MGL_EMPTY;
-    }
-#pragma unroll
-    for (int j = 0; j < MOVE_UNROLL; ++j)
-      if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
-#pragma unroll
-    for (int j = 0; j < MOVE_UNROLL; ++j) {
-      if (u[j] == MGL_EMPTY) continue;
-      const ST wv = wload<M>(A.w, e + (eid)j * stride);
-      const vid cj = r[j].c;
-      const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
-      if (cand) {
-        if (POW2) ins_pow2<ST, AT>(K, V, Av, cj, wv, r[j].a, (int)tab);
-        else ins_any<ST, AT>(K, V, Av, cj, wv, r[j].a, tab);
-      } else if (cj == ci) {
-        if (u[j] != vp) eici += wv; else selfw += wv;
-      }
-    }
-  }
-}
-
-// SHAPE_GROUP: S lanes per vertex, 256/S vertices per workgroup, 2^LG LDS
-// slots per vertex.
-template <class M, class AT, int S, int LG>
-__global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __restrict__ perm,
-                                                    int64_t nb) {
-  typedef typename M::S ST;
-  typedef typename M::K KT;
-  constexpr int GPB = 256 / S;
-  constexpr int H = 1 << LG;
-  __shared__ vid sk[GPB * H];
-  __shared__ ST sv[GPB * H];
-  __shared__ AT sa[GPB * H];
-  __shared__ unsigned long long sdist;
-  const int grp = threadIdx.x / S, lane = threadIdx.x & (S - 1);
-  const int64_t gi = (int64_t)blockIdx.x * GPB + grp;
-  vid* K = sk + grp * H;
-  ST* V = sv + grp * H;
-  AT* Av = sa + grp * H;
-#pragma unroll
-  for (int i = lane; i < H; i += S) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
-  if (threadIdx.x == 0) sdist = 0;
-  vid v = 0;
-  int g = 0;
-  bool ok = gi < nb;
-  if (ok) { v = perm[gi]; ok = vertex_active<M>(A, v, g); }
-  __syncthreads();
-  ST eici = 0, selfw = 0;
-  vid ci = 0;
-  KT acik = 0;
-  if (ok) {
-    const vid vp = rec_pos(A, v);
-    const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
-    ci = rv.c;
-    acik = (KT)rv.a;
-    scan_row<M, AT, true>(A, vp, ci, A.off[v], A.off[v + 1], lane, S, K, V, Av, LG, eici, selfw);
-  }
-  __syncthreads();
-  if (ok) {
-    eici = group_sum<S>(eici);
-    selfw = group_sum<S>(selfw);
-    const double Mh = A.gs[g].mass / 2.0;
-    const KT ki = ((const KT*)A.k)[v];
-    const KT aci_k = acik - ki;
-    const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
-    const bool small = A.tie_small ? true : (A.up != 0);
-    double best = 0.0;
-    vid dst = ci;
-    int cnt = 0;
-#pragma unroll
-    for (int i = lane; i < H; i += S) {
-      const vid cj = K[i];
-      if (cj != MGL_EMPTY) {
-        ++cnt;
-        const double wij = gain((double)V[i], deici, dki, daci, (double)Av[i], Mh);
-        if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
-      }
-    }
-    group_argmax<S>(best, dst, small);
-    if (A.stats) {
-      cnt = group_sum<S>(cnt);
-      if (lane == 0) atomicAdd(&sdist, (unsigned long long)cnt);
-    }
-    if (lane == 0) {
-      A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
-      if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
-    }
-  }
-  if (A.stats) {
-    __syncthreads();
-    if (threadIdx.x == 0 && sdist) atomicAdd(A.stats, sdist);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Workgroup-wide helpers (BS threads, wave64).
-template <int BS, class T>
-__device__ __forceinline__ T block_sum(T v, T* red /* [BS/64] */) {
-  v = group_sum<64>(v);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  __syncthreads();
-  if (l == 0) red[w] = v;
-  __syncthreads();
-  T s = 0;
-#pragma unroll
-  for (int i = 0; i < BS / 64; ++i) s += red[i];
-  return s;
-}
-
-template <int BS>
-__device__ __forceinline__ void block_argmax(double& bg, vid& bd, bool small, double* rg,
-                                             vid* rd) {
-  group_argmax<64>(bg, bd, small);
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  __syncthreads();
-  if (l == 0) { rg[w] = bg; rd[w] = bd; }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < BS / 64; ++i)
-    if (better(rg[i], rd[i], bg, bd, small)) { bg = rg[i]; bd = rd[i]; }
-}
-
-// LDS layout of a table of H slots: keys, values, totals (16-byte aligned parts)
-__host__ __device__ inline size_t tab_bytes(uint32_t H, size_t sv, size_t sa) {
-  return ((4 * (size_t)H + 15) / 16) * 16 + ((sv * H + 15) / 16) * 16 + sa * H;
-}
-
-// SHAPE_BLOCK: one workgroup per vertex, an LDS table of H slots (any H).
-template <class M, class AT, int BS>
-__global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __restrict__ perm,
-                                                   int64_t nb, uint32_t H) {
-  typedef typename M::S ST;
-  typedef typename M::K KT;
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double rg[BS / 64];
-  __shared__ vid rd[BS / 64];
-  __shared__ ST rs[BS / 64];
-  __shared__ int rc[BS / 64];
-  const int64_t gi = blockIdx.x;
-  if (gi >= nb) return;
-  const vid v = perm[gi];
-  int g;
-  if (!vertex_active<M>(A, v, g)) return;  // uniform over the workgroup
-  vid* K = (vid*)smem;
-  ST* V = (ST*)(smem + ((4 * (size_t)H + 15) / 16) * 16);
-  AT* Av = (AT*)((unsigned char*)V + ((sizeof(ST) * (size_t)H + 15) / 16) * 16);
-  for (uint32_t i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
-  const vid vp = rec_pos(A, v);
-  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
-  const vid ci = rv.c;
-  __syncthreads();
-  ST eici = 0, selfw = 0;
-  scan_row<M, AT, false>(A, vp, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, K, V, Av, H, eici,
-                         selfw);
-  __syncthreads();
-  eici = block_sum<BS>(eici, rs);
-  selfw = block_sum<BS>(selfw, rs);
-  const double Mh = A.gs[g].mass / 2.0;
-  const KT ki = ((const KT*)A.k)[v];
-  const KT aci_k = (KT)rv.a - ki;
-  const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
-  const bool small = A.tie_small ? true : (A.up != 0);
-  double best = 0.0;
-  vid dst = ci;
-  int cnt = 0;
-  for (uint32_t i = threadIdx.x; i < H; i += BS) {
-    const vid cj = K[i];
-    if (cj != MGL_EMPTY) {
-      ++cnt;
-      const double wij = gain((double)V[i], deici, dki, daci, (double)Av[i], Mh);
-      if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
-    }
-  }
-  block_argmax<BS>(best, dst, small, rg, rd);
-  if (A.stats) cnt = block_sum<BS>(cnt, rc);
-  if (threadIdx.x == 0) {
-    A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
-    if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
-    if (A.stats) atomicAdd(A.stats, (unsigned long long)cnt);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Gathers and inserts the (at most IT per lane) entries of v's row.
-template <class M, class AT, int IT, class TB>
-__device__ __forceinline__ void own_row(const MoveArgs& A, vid vp, vid ci, eid e0, eid e1,
-                                        int lane, int stride, TB& tb, uint32_t H, bool pow2,
-                                        int lg, int (&own)[IT], AT (&oa)[IT],
-                                        typename M::S& eici, typename M::S& selfw) {
-  typedef typename M::S ST;
-  const Rec<AT>* R = (const Rec<AT>*)A.rec;
-  const bool v1 = A.version == 1;
-  vid u[IT];
-  Rec<AT> r[IT];
-#pragma unroll
-  for (int j = 0; j < IT; ++j) {
-    const eid e = e0 + lane + (eid)j * stride;
-    u[j] = e < e1 ? colload(A.col, e) : MGL_EMPTY;
-  }
-#pragma unroll
-  for (int j = 0; j < IT; ++j)
-    if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
-#pragma unroll
-  for (int j = 0; j < IT; ++j) {
-    own[j] = -1;
-    if (u[j] == MGL_EMPTY) continue;
-    const eid e = e0 + lane + (eid)j * stride;
-    const ST wv = wload<M>(A.w, e);
-    const vid cj = r[j].c;
-    const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
-    if (cand) {
-      own[j] = tb.ins(cj, wv, H, pow2, lg);
-      oa[j] = r[j].a;
-    } else if (cj == ci) {
-      if (u[j] != vp) eici += wv; else selfw += wv;
-    }
-  }
-}
-
-template <class M, class AT, int S, int LG, int IT>
-__global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* __restrict__ perm,
-                                                        int64_t nb) {
-  typedef typename M::S ST;
-  typedef typename M::K KT;
-  constexpr bool PACK = sizeof(ST) == 4;
-  typedef OwnTab<ST, PACK> TB;
-  constexpr int GPB = 256 / S;
-  constexpr int H = 1 << LG;
-  __shared__ __align__(16) unsigned char tabs[GPB * H * TB::slot + 16 * GPB];
-  __shared__ unsigned long long sdist;
-  const int grp = threadIdx.x / S, lane = threadIdx.x & (S - 1);
-  const int64_t gi = (int64_t)blockIdx.x * GPB + grp;
-  TB tb;
-  tb.init(tabs + (size_t)grp * (H * TB::slot + 16), H);
-#pragma unroll
-  for (int i = lane; i < H; i += S) tb.clear(i);
-  if (threadIdx.x == 0) sdist = 0;
-  vid v = 0;
-  int g = 0;
-  bool ok = gi < nb;
-  if (ok) { v = perm[gi]; ok = vertex_active<M>(A, v, g); }
-  __syncthreads();
-  ST eici = 0, selfw = 0;
-  vid ci = 0;
-  KT acik = 0;
-  int own[IT];
-  AT oa[IT];
-#pragma unroll
-  for (int j = 0; j < IT; ++j) own[j] = -1;
-  if (ok) {
-    const vid vp = rec_pos(A, v);
-    const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
-    ci = rv.c;
-    acik = (KT)rv.a;
-    own_row<M, AT, IT, TB>(A, vp, ci, A.off[v], A.off[v + 1], lane, S, tb, H, true, LG, own, oa,
-                           eici, selfw);
-  }
-  __syncthreads();
-  if (ok) {
-    eici = group_sum<S>(eici);
-    selfw = group_sum<S>(selfw);
-    const double Mh = A.gs[g].mass / 2.0;
-    const KT ki = ((const KT*)A.k)[v];
-    const KT aci_k = acik - ki;
-    const bool small = A.tie_small ? true : (A.up != 0);
-    double best = 0.0;
-    vid dst = ci;
-    int cnt = 0;
-    own_gains<M, AT, IT, TB>(tb, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small,
-                             best, dst, cnt);
-    group_argmax<S>(best, dst, small);
-    if (A.stats) {
-      cnt = group_sum<S>(cnt);
-      if (lane == 0) atomicAdd(&sdist, (unsigned long long)cnt);
-    }
-    if (lane == 0) {
-      A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
-      if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
-    }
-  }
-  if (A.stats) {
-    __syncthreads();
-    if (threadIdx.x == 0 && sdist) atomicAdd(A.stats, sdist);
-  }
-}
-
-template <class M, class AT, int BS, int IT>
-__global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __restrict__ perm,
-                                                       int64_t nb, uint32_t H) {
-  typedef typename M::S ST;
-  typedef typename M::K KT;
-  constexpr bool PACK = sizeof(ST) == 4;
-  typedef OwnTab<ST, PACK> TB;
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double rg[BS / 64];
-  __shared__ vid rd[BS / 64];
-  __shared__ ST rs[BS / 64];
-  __shared__ int rc[BS / 64];
-  const int64_t gi = blockIdx.x;
-  if (gi >= nb) return;
-  const vid v = perm[gi];
-  int g;
-  if (!vertex_active<M>(A, v, g)) return;  // uniform over the workgroup
-  TB tb;
-  tb.init(smem, H);
-  for (uint32_t i = threadIdx.x; i < H; i += BS) tb.clear(i);
-  const vid vp = rec_pos(A, v);
-  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
-  const vid ci = rv.c;
-  __syncthreads();
-  ST eici = 0, selfw = 0;
-  int own[IT];
-  AT oa[IT];
-  own_row<M, AT, IT, TB>(A, vp, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, tb, H, false, 0, own,
-                         oa, eici, selfw);
-  __syncthreads();
-  eici = block_sum<BS>(eici, rs);
-  selfw = block_sum<BS>(selfw, rs);
-  const double Mh = A.gs[g].mass / 2.0;
-  const KT ki = ((const KT*)A.k)[v];
-  const KT aci_k = (KT)rv.a - ki;
-  const bool small = A.tie_small ? true : (A.up != 0);
-  double best = 0.0;
-  vid dst = ci;
-  int cnt = 0;
-  own_gains<M, AT, IT, TB>(tb, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small, best,
-                           dst, cnt);
-  block_argmax<BS>(best, dst, small, rg, rd);
-  if (A.stats) cnt = block_sum<BS>(cnt, rc);
-  if (threadIdx.x == 0) {
-    A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
-    if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
-    if (A.stats) atomicAdd(A.stats, (unsigned long long)cnt);
-  }
-}
-
-// ---------------------------------------------------------------------------
-// SHAPE_LANE: one lane per vertex for rows of at most D entries. The row is
-// gathered into registers and equal communities are merged by pairwise
-// comparison (the first occurrence accumulates), so no LDS, barrier or
-// cross-lane operation is involved.
-template <class M, class AT, int D>
-__global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __restrict__ perm,
-                                                   int64_t nb) {
-  typedef typename M::S ST;
-  typedef typename M::K KT;
-  __shared__ unsigned long long sdist;
-  if (threadIdx.x == 0) sdist = 0;
-  if (A.stats) __syncthreads();
-  const int64_t gi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  int g = 0;
-  vid v = 0;
-  bool ok = gi < nb;
-  if (ok) { v = perm[gi]; ok = vertex_active<M>(A, v, g); }
-  int cnt = 0;
-  if (ok) {
-    const Rec<AT>* R = (const Rec<AT>*)A.rec;
-    const vid vp = rec_pos(A, v);
-    const Rec<AT> rv = R[vp];
-    const vid ci = rv.c;
-    const eid e0 = A.off[v];
-    const int d = (int)(A.off[v + 1] - e0);
-    vid u[D];
-    Rec<AT> r[D];
-    ST w[D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) u[j] = j < d ? colload(A.col, e0 + j) : MGL_EMPTY;
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
-      w[j] = (u[j] != MGL_EMPTY) ? wload<M>(A.w, e0 + j) : (ST)0;
-    }
-    const bool v1 = A.version == 1;
-    ST eici = 0, selfw = 0;
-    bool cand[D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      cand[j] = false;
-      if (u[j] == MGL_EMPTY) continue;
-      const vid cj = r[j].c;
-      cand[j] = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
-      if (!cand[j] && cj == ci) {
-        if (u[j] != vp) eici += w[j]; else selfw += w[j];
-      }
-    }
-    // merge duplicates into their first occurrence
-#pragma unroll
-    for (int j = 1; j < D; ++j) {
-#pragma unroll
-      for (int i = 0; i < j; ++i) {
-        if (cand[i] && cand[j] && r[i].c == r[j].c) {
-          w[i] += w[j];
-          cand[j] = false;
-        }
-      }
-    }
-    const double Mh = A.gs[g].mass / 2.0;
-    const KT ki = ((const KT*)A.k)[v];
-    const KT aci_k = (KT)rv.a - ki;
-    const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
-    const bool small = A.tie_small ? true : (A.up != 0);
-    double best = 0.0;
-    vid dst = ci;
-#pragma unroll
-    for (int j = 0; j < D; ++j) {
-      if (!cand[j]) continue;
-      ++cnt;
-      const double wij = gain((double)w[j], deici, dki, daci, (double)r[j].a, Mh);
-      if (better(wij, r[j].c, best, dst, small)) { best = wij; dst = r[j].c; }
-    }
-    A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
-    if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
-  }
-  if (A.stats) {
-    cnt = group_sum<64>(cnt);
-    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&sdist, (unsigned long long)cnt);
-    __syncthreads();
-    if (threadIdx.x == 0 && sdist) atomicAdd(A.stats, sdist);
-  }
-}
-
 // ---------------------------------------------------------------------------
 BinLayout default_bins(int lds_bytes, int slot_bytes) {
   BinLayout L;
@@ -620,108 +139,38 @@ BinLayout default_bins(int lds_bytes, int slot_bytes) {
   return L;
 }
 
-template <class M, class AT>
-static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int64_t nb,
-                      hipStream_t st) {
-  typedef typename M::S ST;
-  if (nb <= 0 || bs.shape == SHAPE_NONE) return 0;
-  if (bs.shape == SHAPE_LANE) {
-    const unsigned grid = (unsigned)((nb + 255) / 256);
-    if (bs.upper <= 4) { k_move_lane<M, AT, 4><<<grid, 256, 0, st>>>(A, perm, nb); return 0; }
-    if (bs.upper <= 8) { k_move_lane<M, AT, 8><<<grid, 256, 0, st>>>(A, perm, nb); return 0; }
-    if (bs.upper <= 16) { k_move_lane<M, AT, 16><<<grid, 256, 0, st>>>(A, perm, nb); return 0; }
-    return -1;
-  }
-  if (bs.shape == SHAPE_GROUP) {
-    const int gpb = 256 / bs.s;
-    const unsigned grid = (unsigned)((nb + gpb - 1) / gpb);
-#define G_(S_, LG_)                                                                     \
-  if (bs.s == S_ && bs.size == LG_) {                                                   \
-    k_move_group<M, AT, S_, LG_><<<grid, 256, 0, st>>>(A, perm, nb);                    \
-    return 0;                                                                           \
-  }
-// owner-based instances (lanes S, table 2^LG, entries per lane IT); the
-// smallest IT that covers the bin is taken, longer rows use the scan kernel
-#define O_(S_, LG_, IT_)                                                                \
-  if (bs.s == S_ && bs.size == LG_ && need <= IT_) {                                    \
-    k_move_group_own<M, AT, S_, LG_, IT_><<<grid, 256, 0, st>>>(A, perm, nb);           \
-    return 0;                                                                           \
-  }
-    const int64_t need = (bs.upper + bs.s - 1) / bs.s;
-    O_(4, 3, 1) O_(4, 3, 2) O_(4, 4, 4) O_(4, 5, 4)
-    O_(8, 4, 1) O_(8, 5, 2) O_(8, 5, 4) O_(8, 6, 4)
-    O_(16, 5, 1) O_(16, 6, 2) O_(16, 6, 4) O_(16, 7, 4)
-    O_(32, 6, 1) O_(32, 7, 2) O_(32, 7, 4) O_(32, 8, 4)
-    O_(64, 7, 1) O_(64, 8, 2) O_(64, 8, 4) O_(64, 9, 4) O_(64, 9, 8) O_(64, 10, 8)
-    G_(4, 3) G_(4, 4) G_(8, 4) G_(8, 5) G_(16, 5) G_(16, 6) G_(32, 6) G_(32, 7)
-    G_(64, 7) G_(64, 8) G_(64, 9) G_(64, 10)
-#undef O_
-#undef G_
-  } else if (bs.shape == SHAPE_BLOCK) {
-    const uint32_t H = (uint32_t)bs.size;
-    const size_t lds_own = (sizeof(ST) == 4) ? 8 * (size_t)H
-                                             : ((4 * (size_t)H + 15) / 16) * 16 + sizeof(ST) * (size_t)H;
-    const int64_t it = (bs.upper + bs.s - 1) / bs.s;
-#define B_(BS_, IT_)                                                                        \
-  if (bs.s == BS_ && it <= IT_) {                                                           \
-    k_move_block_own<M, AT, BS_, IT_><<<(unsigned)nb, BS_, lds_own, st>>>(A, perm, nb, H);  \
-    return 0;                                                                               \
-  }
-    B_(256, 1) B_(256, 2) B_(256, 4) B_(256, 8) B_(256, 16)
-    B_(512, 1) B_(512, 2) B_(512, 4) B_(512, 8) B_(512, 16)
-    B_(1024, 1) B_(1024, 2) B_(1024, 4) B_(1024, 8) B_(1024, 16)
-#undef B_
-    const size_t lds = tab_bytes(H, sizeof(ST), sizeof(AT));
-    if (bs.s == 256) {
-      k_move_block<M, AT, 256><<<(unsigned)nb, 256, lds, st>>>(A, perm, nb, H);
-      return 0;
-    }
-    if (bs.s == 512) {
-      k_move_block<M, AT, 512><<<(unsigned)nb, 512, lds, st>>>(A, perm, nb, H);
-      return 0;
-    }
-    if (bs.s == 1024) {
-      k_move_block<M, AT, 1024><<<(unsigned)nb, 1024, lds, st>>>(A, perm, nb, H);
-      return 0;
-    }
+// ---------------------------------------------------------------------------
+// Move-kernel dispatch over the per-mode instantiation units (mgl_move_*.hip).
+static int move_bin_any(int mode, int rec32, bool cnt, const BinSpec& bs, const MoveArgs& A,
+                        const vid* perm, int64_t nb, hipStream_t st, char* name, int len) {
+  switch (mode) {
+    case MODE_UNIT: return move_bin_unit(mode, rec32, cnt, bs, A, perm, nb, st, name, len);
+    case MODE_I32: return move_bin_i32(mode, rec32, cnt, bs, A, perm, nb, st, name, len);
+    case MODE_INT:
+    case MODE_F64: return move_bin_int(mode, rec32, cnt, bs, A, perm, nb, st, name, len);
+    case MODE_F32: return move_bin_f32(mode, rec32, cnt, bs, A, perm, nb, st, name, len);
   }
   return -1;
+}
+
+int move_kernel_name(int mode, int rec32, const BinSpec& bs, char* buf, int len, bool cnt) {
+  if (!buf || len <= 0) return -1;
+  buf[0] = 0;
+  MoveArgs A{};
+  return move_bin_any(mode, rec32, cnt, bs, A, nullptr, 1, 0, buf, len);
 }
 
 int launch_move_bin(int mode, int rec32, const BinSpec& bs, const MoveArgs& A, const vid* perm,
                     int64_t nb, hipStream_t st) {
-  switch (mode) {
-    case MODE_UNIT:
-      return rec32 ? move_bin_t<ModeUnit, int32_t>(bs, A, perm, nb, st)
-                   : move_bin_t<ModeUnit, int64_t>(bs, A, perm, nb, st);
-    case MODE_INT: return move_bin_t<ModeInt, int64_t>(bs, A, perm, nb, st);
-    case MODE_I32:
-      return rec32 ? move_bin_t<ModeI32, int32_t>(bs, A, perm, nb, st)
-                   : move_bin_t<ModeI32, int64_t>(bs, A, perm, nb, st);
-    case MODE_F64: return move_bin_t<ModeF64, double>(bs, A, perm, nb, st);
-    case MODE_F32: return move_bin_t<ModeF32, double>(bs, A, perm, nb, st);
-  }
-  return -1;
+  const bool cnt = A.stats != nullptr || A.probes != nullptr;
+  return move_bin_any(mode, rec32, cnt, bs, A, perm, nb, st, nullptr, 0);
 }
 
-// Opt the LDS-table kernels into the device's largest workgroup LDS size.
 void move_kernels_allow_lds(int bytes) {
-#define A_(M_, AT_, BS_) \
-  (void)hipFuncSetAttribute((const void*)k_move_block<M_, AT_, BS_>, \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-#define O_(M_, AT_, BS_, IT_) \
-  (void)hipFuncSetAttribute((const void*)k_move_block_own<M_, AT_, BS_, IT_>, \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-#define P_(M_, AT_, BS_) O_(M_, AT_, BS_, 1) O_(M_, AT_, BS_, 2) O_(M_, AT_, BS_, 4) \
-  O_(M_, AT_, BS_, 8) O_(M_, AT_, BS_, 16)
-#define B_(M_, AT_) A_(M_, AT_, 256) A_(M_, AT_, 512) A_(M_, AT_, 1024) \
-  P_(M_, AT_, 256) P_(M_, AT_, 512) P_(M_, AT_, 1024)
-  B_(ModeUnit, int32_t) B_(ModeUnit, int64_t) B_(ModeInt, int64_t) B_(ModeF64, double)
-  B_(ModeF32, double) B_(ModeI32, int32_t) B_(ModeI32, int64_t)
-#undef B_
-#undef P_
-#undef O_
-#undef A_
+  move_allow_lds_unit(bytes);
+  move_allow_lds_i32(bytes);
+  move_allow_lds_int(bytes);
+  move_allow_lds_f32(bytes);
 }
 
 // ---------------------------------------------------------------------------

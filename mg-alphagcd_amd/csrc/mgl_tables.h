@@ -23,6 +23,29 @@ __device__ __forceinline__ double gain(double e, double eici, double ki, double 
   return wij;
 }
 
+// Per-thread statistics of LDS hash inserts: inserts, compare-and-swap probes
+// and inserts that needed more than one probe. Summed per wavefront and added
+// to A.probes[0..2] when statistics are requested.
+struct ProbeCount {
+  uint32_t ins = 0, probes = 0, multi = 0;
+  __device__ __forceinline__ void add(uint32_t np) {
+    ++ins;
+    probes += np;
+    multi += np > 1u ? 1u : 0u;
+  }
+};
+// Every lane of the wavefront must call this (shuffle reduction).
+__device__ __forceinline__ void flush_probes(unsigned long long* out, const ProbeCount& pc) {
+  if (!out) return;
+  const uint32_t a = group_sum<64>(pc.ins), b = group_sum<64>(pc.probes),
+                 c = group_sum<64>(pc.multi);
+  if ((threadIdx.x & 63) == 0 && a) {
+    atomicAdd(out, (unsigned long long)a);
+    atomicAdd(out + 1, (unsigned long long)b);
+    atomicAdd(out + 2, (unsigned long long)c);
+  }
+}
+
 // Owner-based variants (rows of at most IT entries per lane): each lane keeps
 // in registers the entries it gathered and the slots whose key it created.
 // The table holds key and partial sum only; after the barrier every creator
@@ -33,7 +56,7 @@ __device__ __forceinline__ double gain(double e, double eici, double ki, double 
 // stores its first weight; a known key is one 32-bit add on the low half.
 template <class ST>
 __device__ __forceinline__ int own_packed(uint64_t* T, vid key, ST w, uint32_t H, bool pow2,
-                                          int lg) {
+                                          int lg, ProbeCount& pc) {
   uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
   const uint32_t st = probe_step(key, H, pow2, lg);
   uint32_t wb;
@@ -43,13 +66,15 @@ __device__ __forceinline__ int own_packed(uint64_t* T, vid key, ST w, uint32_t H
   for (uint32_t i = 0; i < H; ++i) {
     const uint64_t old = atomicCAS((unsigned long long*)&T[h], (unsigned long long)empty,
                                    (unsigned long long)want);
-    if (old == empty) return (int)h;
+    if (old == empty) { pc.add(i + 1); return (int)h; }
     if ((uint32_t)(old >> 32) == (uint32_t)key) {
       atom_add((ST*)&T[h], w);  // the low half
+      pc.add(i + 1);
       return -1;
     }
     h = probe_next(h, st, H);
   }
+  pc.add(H);
   return -1;
 }
 
@@ -57,17 +82,19 @@ __device__ __forceinline__ int own_packed(uint64_t* T, vid key, ST w, uint32_t H
 // creator; the sum is added by everyone.
 template <class ST>
 __device__ __forceinline__ int own_sep(vid* K, ST* V, vid key, ST w, uint32_t H, bool pow2,
-                                       int lg) {
+                                       int lg, ProbeCount& pc) {
   uint32_t h = pow2 ? slot_hash(key, lg) : any_slot(key, H);
   const uint32_t st = probe_step(key, H, pow2, lg);
   for (uint32_t i = 0; i < H; ++i) {
     const vid cur = atomicCAS(&K[h], MGL_EMPTY, key);
     if (cur == MGL_EMPTY || cur == key) {
       atom_add(&V[h], w);
+      pc.add(i + 1);
       return cur == MGL_EMPTY ? (int)h : -1;
     }
     h = probe_next(h, st, H);
   }
+  pc.add(H);
   return -1;
 }
 
@@ -80,8 +107,8 @@ struct OwnTab<ST, true> {
   static constexpr size_t slot = 8;
   __device__ void init(unsigned char* base, uint32_t) { T = (uint64_t*)base; }
   __device__ void clear(uint32_t i) { T[i] = (uint64_t)(uint32_t)MGL_EMPTY << 32; }
-  __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg) {
-    return own_packed<ST>(T, key, w, H, pow2, lg);
+  __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg, ProbeCount& pc) {
+    return own_packed<ST>(T, key, w, H, pow2, lg, pc);
   }
   __device__ vid key(int i) const { return (vid)(uint32_t)(T[i] >> 32); }
   __device__ ST val(int i) const {
@@ -101,8 +128,8 @@ struct OwnTab<ST, false> {
     V = (ST*)(base + ((4 * (size_t)H + 15) / 16) * 16);
   }
   __device__ void clear(uint32_t i) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
-  __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg) {
-    return own_sep<ST>(K, V, key, w, H, pow2, lg);
+  __device__ int ins(vid key, ST w, uint32_t H, bool pow2, int lg, ProbeCount& pc) {
+    return own_sep<ST>(K, V, key, w, H, pow2, lg, pc);
   }
   __device__ vid key(int i) const { return K[i]; }
   __device__ ST val(int i) const { return V[i]; }

@@ -498,4 +498,21 @@ extern "C" double orc_pass_i64(i64 n, const i64* off, const i32* col, const i64*
   return q;
 }
 
+// The same pass in FP64 (weights, k, a as doubles; the reference's weight_t),
+// the CPU baseline of weighted graphs whose weights are not dyadic.
+extern "C" void orc_vertex_weights_f64(i64 n, const i64* off, const double* w, double* k) {
+  G<double> g{n, off, nullptr, w};
+  vertex_weights(g, k);
+}
+extern "C" double orc_pass_f64(i64 n, const i64* off, const i32* col, const double* w,
+                               const double* k, i32* c, double* a, double mass, int up,
+                               int version, i32* c_new, double cur_q, double thr) {
+  G<double> g{n, off, col, w};
+  move_pass(g, k, c, a, mass, up, version, c_new);
+  update_weights(n, k, c, c_new, a);
+  double q = modularity_fp64(g, c_new, a, mass);
+  if (q - cur_q > thr) std::memcpy(c, c_new, sizeof(i32) * n);
+  return q;
+}
+
 extern "C" int orc_num_threads() { return omp_get_max_threads(); }

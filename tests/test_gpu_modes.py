@@ -113,3 +113,35 @@ def test_rmat24_full_run_properties(ctx, mgl, orc):
     at, kt = s.weights()
     assert at.sum() == g.nnz and kt.sum() == g.nnz
     s.close()
+
+
+def test_session_count_matches_numpy_and_leaves_passes_unchanged(ctx, mgl):
+    g = mgl.gen_rmat(13, 16, seed=17)
+    # distinct (vertex, neighbour community != own) pairs of a random assignment
+    rng = np.random.default_rng(4)
+    c = rng.integers(0, g.n // 16, g.n).astype(np.int32) * 16
+    s = ctx.session(g, fuse_modularity=False)
+    s.set_assign(c)
+    s.count()
+    s.pass_()
+    st = s.stats()
+    rows = np.repeat(np.arange(g.n, dtype=np.int64), np.diff(g.offsets))
+    cu = c[g.cols].astype(np.int64)
+    keep = cu != c[rows]
+    want = len(np.unique(rows[keep] * g.n + cu[keep]))
+    assert st["distinct"] == want
+    assert 0 < st["lds_inserts"] <= st["lds_probes"]
+    assert 0 <= st["lds_multi_probe"] <= st["lds_inserts"]
+    s.close()
+    # counting before each pass does not change what the passes compute
+    a, b = ctx.session(g), ctx.session(g)
+    a.reset()
+    b.reset()
+    for _ in range(6):
+        b.count()
+        qa, acc_a = a.pass_()
+        qb, acc_b = b.pass_()
+        assert qa == qb and acc_a == acc_b
+        assert b.stats()["distinct"] > 0 and a.stats()["distinct"] == 0
+    a.close()
+    b.close()
