@@ -29,14 +29,22 @@
 #include <vector>
 
 // ---------------------------------------------------------------------------
+// Bin layout and hub-path settings of a run: the built-in ones, or a tuning
+// file's (mgl_opts.tuning_file), which applies to the call or session that
+// names it only.
+struct Tuning {
+  BinLayout bins;
+  bool have_bins = false;     // false: built-in layout for the device
+  int64_t hub_bucket = 4096;  // hub candidates per bucket on average
+  int hub_segmented = 1;      // 0: the hub path always scatters into buckets
+};
+
 struct mgl_ctx {
   int device = 0;
   hipStream_t st = nullptr;
   std::string err;
   int lds_max = 65536;
-  BinLayout user_bins;
-  bool have_user_bins = false;
-  int64_t hub_bucket = 4096;  // hub candidates per bucket on average
+  Tuning tune;  // settings of calls without a tuning file (the autotuner varies them)
 };
 
 #define FAIL(ctx, code, ...)                                    \
@@ -132,9 +140,9 @@ static int prime_at_most(int x) {
 // Bin layout for a mode, with LDS tables shrunk to what the device allows.
 // BLOCK bins of at most 16 entries per thread use the owner-based kernel
 // (key + partial sum per slot); longer ones also keep the total in the slot.
-static BinLayout layout_for(const mgl_ctx* ctx, int mode, bool rec32) {
+static BinLayout layout_for(const mgl_ctx* ctx, const Tuning& tu, int mode, bool rec32) {
   const int sb = 4 + slot_bytes(mode);
-  BinLayout L = ctx->have_user_bins ? ctx->user_bins : default_bins(ctx->lds_max, sb);
+  BinLayout L = tu.have_bins ? tu.bins : default_bins(ctx->lds_max, sb);
   for (int i = 0; i < L.nb; ++i) {
     BinSpec& b = L.b[i];
     if (b.shape == SHAPE_BLOCK) {
@@ -221,7 +229,7 @@ struct Engine {
   std::vector<int64_t> bin_n, bin_off;
   // hub plan of the current level (last bin)
   DBuf hchunk, hv, hbk, hbkhub, htmpc, htmpw, hbktc, hbktw, hbcount, hboff, hbcur, heici, hself,
-      hbgain, hbdst, hgk, hgv, hscan, htmpa, hbkta, hga;
+      hbgain, hbdst, hgk, hgv, hscan, htmpa, hbkta, hga, hchunk0, hcbo0, hcbo, hgalloc;
   HubPlan hub;
   // coarsening buffers
   DBuf cflag, crank, cnewc, ck0, ck1, cv0, cv1, chist, chead, cuidx, crowcnt, cscan;
@@ -240,7 +248,9 @@ struct Engine {
   bool timing = false;    // HIP events around each bin and stage
   bool counting = false;  // per-bin distinct pairs and LDS probe counters (global atomics)
 
-  explicit Engine(mgl_ctx* c_) : ctx(c_) {
+  Tuning tune;  // this run's bin layout and hub settings
+
+  explicit Engine(mgl_ctx* c_) : ctx(c_), tune(c_->tune) {
     for (auto& e : ev) e = nullptr;
   }
   ~Engine() {
@@ -326,7 +336,7 @@ struct Engine {
   // Binning of the current level (BIN::bin_create, BIN.cu:468-499).
   mgl_status make_bins() {
     LevelGraph& g = LG();
-    L = layout_for(ctx, g.mode, rec32_now());
+    L = layout_for(ctx, tune, g.mode, rec32_now());
     const int nb = L.nb;
     bin_n.assign(nb, 0);
     bin_off.assign(nb + 1, 0);
@@ -371,11 +381,12 @@ struct Engine {
       HIPC(ctx, hipMemcpyAsync(ranges.data(), rr.p, 16 * m, hipMemcpyDeviceToHost, st()));
       HIPC(ctx, hipStreamSynchronize(st()));
       const int64_t CH = hub_chunk_edges();
-      const int64_t BK = ctx->hub_bucket;
-      std::vector<int64_t> chunk, hbkv(m + 1, 0), bkhub;
-      int64_t pos = 0;
+      const int64_t BK = tune.hub_bucket;
+      std::vector<int64_t> chunk, hbkv(m + 1, 0), bkhub, chunk0(m + 1, 0), cbo0(m, 0);
+      int64_t pos = 0, ncbo = 0, max_nch = 0;
       for (int64_t i = 0; i < m; ++i) {
         const int64_t e0 = ranges[2 * i], e1 = ranges[2 * i + 1], d = e1 - e0;
+        chunk0[i] = (int64_t)chunk.size() / 3;
         for (int64_t e = e0; e < e1; e += CH) {
           chunk.push_back(i);
           chunk.push_back(e);
@@ -387,7 +398,15 @@ struct Engine {
         if (P > hub_max_buckets()) P = hub_max_buckets();
         hbkv[i + 1] = hbkv[i] + P;
         for (int64_t j = 0; j < P; ++j) bkhub.push_back(i);
+        const int64_t nch = (int64_t)chunk.size() / 3 - chunk0[i];
+        cbo0[i] = ncbo;
+        ncbo += nch * (P + 1);
+        max_nch = std::max(max_nch, nch);
       }
+      chunk0[m] = (int64_t)chunk.size() / 3;
+      // segmented form unless a hub has more chunks than the aggregate's LDS
+      // segment table holds (degree above hub_seg_max() chunks)
+      hub.segmented = (tune.hub_segmented && max_nch <= hub_seg_max()) ? 1 : 0;
       hub.m = m;
       hub.nchunks = (int64_t)chunk.size() / 3;
       hub.nbuckets = hbkv[m];
@@ -409,6 +428,10 @@ struct Engine {
         HIPC(ctx, hbktw.ensure(sb * e));
       }
       HIPC(ctx, hbcount.ensure(8 * hub.nbuckets));
+      HIPC(ctx, hchunk0.ensure(8 * (m + 1)));
+      HIPC(ctx, hcbo0.ensure(8 * m));
+      HIPC(ctx, hcbo.ensure(4 * std::max<int64_t>(ncbo, 1)));
+      HIPC(ctx, hgalloc.ensure(8));
       HIPC(ctx, hboff.ensure(8 * (hub.nbuckets + 1)));
       HIPC(ctx, hbcur.ensure(8 * hub.nbuckets));
       HIPC(ctx, heici.ensure(8 * m));
@@ -422,6 +445,8 @@ struct Engine {
       HIPC(ctx, hipMemcpyAsync(hv.p, hperm, 4 * m, hipMemcpyDeviceToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hbk.p, hbkv.data(), 8 * (m + 1), hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hbkhub.p, bkhub.data(), 8 * hub.nbuckets, hipMemcpyHostToDevice, st()));
+      HIPC(ctx, hipMemcpyAsync(hchunk0.p, chunk0.data(), 8 * (m + 1), hipMemcpyHostToDevice, st()));
+      HIPC(ctx, hipMemcpyAsync(hcbo0.p, cbo0.data(), 8 * m, hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipStreamSynchronize(st()));
       hub.chunk = hchunk.as<int64_t>();
       hub.hub_v = hv.as<vid>();
@@ -444,11 +469,18 @@ struct Engine {
       hub.gk = hgk.as<vid>();
       hub.gv = hgv.p;
       hub.scan_tmp = hscan.p;
+      hub.hub_chunk0 = hchunk0.as<int64_t>();
+      hub.hub_cbo0 = hcbo0.as<int64_t>();
+      hub.cbo = hcbo.as<int32_t>();
+      hub.galloc = hgalloc.as<unsigned long long>();
       // owner tables of the aggregate: (key, sum) slots in half of the
       // workgroup LDS, so that two buckets share a CU; prime-sized
       {
         const int sb = slot_bytes(g.mode) == 4 ? 8 : 4 + slot_bytes(g.mode);
-        hub.lds_slots = (uint32_t)prime_at_most((int)((ctx->lds_max / 2 - 1024) / sb));
+        // the segmented aggregate also holds its hub's segment table (two
+        // 4-byte entries per chunk) in static LDS
+        const int seg = hub.segmented ? 8 * hub_seg_max() + 64 : 0;
+        hub.lds_slots = (uint32_t)prime_at_most((int)((ctx->lds_max / 2 - 1024 - seg) / sb));
       }
     }
     return MGL_OK;
@@ -1398,7 +1430,7 @@ void mgl_opts_default(mgl_opts* o) {
 
 int mgl_version_major(void) { return 1; }
 
-static mgl_status load_tuning(mgl_ctx* ctx, const char* path);
+static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu);
 
 mgl_status mgl_create(mgl_ctx** out, int device) {
   if (!out) return MGL_INVALID_ARG;
@@ -1460,13 +1492,13 @@ mgl_status mgl_louvain_batched(mgl_ctx* ctx, const mgl_csr* graphs, int32_t n_gr
   else mgl_opts_default(&o);
   mgl_status s = check_opts(ctx, &o);
   if (s) return s;
-  if (o.tuning_file) {
-    s = load_tuning(ctx, o.tuning_file);
-    if (s) return s;
-  }
   for (int i = 0; i < n_graphs; ++i) results[i] = nullptr;
   Engine E(ctx);
   E.o = o;
+  if (o.tuning_file) {
+    s = load_tuning(ctx, o.tuning_file, E.tune);
+    if (s) return s;
+  }
   s = E.setup_union(graphs, n_graphs);
   if (s) return s;
   s = E.alloc_coarsen();
@@ -1534,7 +1566,7 @@ void mgl_result_free(mgl_result* r) { delete r; }
 
 int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins) {
   if (!ctx) return 0;
-  BinLayout L = layout_for(ctx, MODE_UNIT, true);
+  BinLayout L = layout_for(ctx, ctx->tune, MODE_UNIT, true);
   for (int i = 0; i < L.nb && i < max_bins; ++i) upper[i] = L.b[i].upper;
   return L.nb;
 }
@@ -1634,14 +1666,14 @@ mgl_status mgl_session_create(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* op
   else mgl_opts_default(&o);
   mgl_status s = check_opts(ctx, &o);
   if (s) return s;
-  if (o.tuning_file) {
-    s = load_tuning(ctx, o.tuning_file);
-    if (s) return s;
-  }
   mgl_session* ss = new mgl_session();
   ss->ctx = ctx;
   ss->E = new Engine(ctx);
   ss->E->o = o;
+  if (o.tuning_file) {
+    s = load_tuning(ctx, o.tuning_file, ss->E->tune);
+    if (s) { delete ss->E; delete ss; return s; }
+  }
   s = ss->E->setup_union(g, 1);
   if (s) { delete ss->E; delete ss; return s; }
   ss->E->running.assign(1, 1);
@@ -1992,7 +2024,7 @@ mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64
 // ---------------------------------------------------------------------------
 // Tuning file: one line per bin, "upper shape s lg" (text). Written by
 // mgl_autotune, read at context use.
-static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
+static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
   FILE* f = fopen(path, "r");
   if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot open tuning file %s", path);
   BinLayout L;
@@ -2003,7 +2035,12 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
     long long hb;
     if (sscanf(line, "hub_bucket %lld", &hb) == 1) {
       if (hb < 1) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_bucket", path);
-      ctx->hub_bucket = hb;
+      tu.hub_bucket = hb;
+      continue;
+    }
+    int hs;
+    if (sscanf(line, "hub_segmented %d", &hs) == 1) {
+      tu.hub_segmented = hs != 0;
       continue;
     }
     long long up;
@@ -2033,8 +2070,8 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path) {
   }
   if (L.b[L.nb - 1].shape != SHAPE_GLOBAL)
     FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: last bin must use the HBM hash", path);
-  ctx->user_bins = L;
-  ctx->have_user_bins = true;
+  tu.bins = L;
+  tu.have_bins = true;
   return MGL_OK;
 }
 
@@ -2098,18 +2135,17 @@ static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
 extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* path) {
   if (!ctx) return MGL_INVALID_ARG;
   if (!g || !path) FAIL(ctx, MGL_INVALID_ARG, "null argument");
-  const BinLayout saved = ctx->user_bins;
-  const bool had = ctx->have_user_bins;
+  const Tuning saved = ctx->tune;
   double best_ms = 1e300;
   BinLayout best = default_bins(ctx->lds_max, 4 + slot_bytes(MODE_UNIT));
   for (const BinLayout& cand : tuning_candidates(ctx)) {
-    ctx->user_bins = cand;
-    ctx->have_user_bins = true;
+    ctx->tune.bins = cand;
+    ctx->tune.have_bins = true;
     mgl_opts o;
     mgl_opts_default(&o);
     mgl_session* s = nullptr;
     mgl_status st = mgl_session_create(ctx, g, &o, &s);
-    if (st) { ctx->user_bins = saved; ctx->have_user_bins = had; return st; }
+    if (st) { ctx->tune = saved; return st; }
     double q;
     int acc;
     std::vector<double> t;
@@ -2119,13 +2155,12 @@ extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* p
       if (st == MGL_OK && i > 0) t.push_back(s->st.move_ms);
     }
     mgl_session_destroy(s);
-    if (st) { ctx->user_bins = saved; ctx->have_user_bins = had; return st; }
+    if (st) { ctx->tune = saved; return st; }
     std::sort(t.begin(), t.end());
     const double med = t.empty() ? 1e300 : t[t.size() / 2];
     if (med < best_ms) { best_ms = med; best = cand; }
   }
-  ctx->user_bins = saved;
-  ctx->have_user_bins = had;
+  ctx->tune = saved;
   FILE* f = fopen(path, "w");
   if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot write %s", path);
   fprintf(f,

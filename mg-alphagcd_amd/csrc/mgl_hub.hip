@@ -26,6 +26,7 @@
 #define HUB_ITEMS (HUB_CHUNK / HUB_T)
 #define HUB_MAXP 1024
 #define HUB_AGG_IT 8
+#define HUB_SEG_MAX 2048  // chunks per hub in the segmented form
 
 __device__ __forceinline__ uint32_t hub_bucket(vid key, uint32_t P) {
   return (uint32_t)(((uint64_t)((uint32_t)key * 2654435761u) * P) >> 32);
@@ -129,6 +130,121 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
     if (hist[b]) atomicAdd((unsigned long long*)&H.bcount[H.hub_bk[i] + b], (unsigned long long)hist[b]);
 }
 
+// ---- segmented form: head and count --------------------------------------------
+// Per pass: zero the hubs' own-community sums and the HBM-table pool.
+__global__ void k_hub_head(HubPlan H) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *H.galloc = 0;
+  if (i < H.m) {
+    ((int64_t*)H.hub_eici)[i] = 0;  // 8-byte sums (int64 or double zero)
+    ((int64_t*)H.hub_self)[i] = 0;
+  }
+}
+
+// Gathers c[u] once per entry of a chunk, sums the own community (eici,
+// self-loop), and writes the chunk's candidates into its region of tmp grouped
+// by bucket (counting sort in LDS); cbo receives the chunk's bucket offsets.
+template <class M, class AT>
+__global__ __launch_bounds__(HUB_T) void k_hub_count_seg(MoveArgs A, HubPlan H) {
+  typedef typename M::S ST;
+  typedef typename M::K KT;
+  __shared__ unsigned int hist[HUB_MAXP + 1];
+  __shared__ unsigned int wsum[HUB_T / 64];
+  __shared__ KT red[HUB_T / 64];
+  const int64_t ch = blockIdx.x;
+  const int64_t i = H.chunk[3 * ch];
+  const int64_t e0 = H.chunk[3 * ch + 1];
+  const int64_t pos0 = H.chunk[3 * ch + 2];
+  const vid v = H.hub_v[i];
+  const int g = A.gid ? A.gid[v] : 0;
+  if (!A.gs[g].active) return;
+  const int64_t rend = A.off[v + 1];
+  const int64_t e1 = (e0 + HUB_CHUNK < rend) ? e0 + HUB_CHUNK : rend;
+  const uint32_t P = (uint32_t)(H.hub_bk[i + 1] - H.hub_bk[i]);
+  for (uint32_t b = threadIdx.x; b <= P; b += HUB_T) hist[b] = 0;
+  const Rec<AT>* R = (const Rec<AT>*)A.rec;
+  const vid vp = A.rpos ? A.rpos[v] : v;
+  const vid ci = R[vp].c;
+  const bool v1 = A.version == 1;
+  vid key[HUB_ITEMS];
+  Rec<AT> r[HUB_ITEMS];
+#pragma unroll
+  for (int j = 0; j < HUB_ITEMS; ++j) {
+    const int64_t e = e0 + (int64_t)j * HUB_T + threadIdx.x;
+    key[j] = e < e1 ? colload(A.col, e) : MGL_EMPTY;  // neighbour position first
+  }
+#pragma unroll
+  for (int j = 0; j < HUB_ITEMS; ++j)
+    if (key[j] != MGL_EMPTY) r[j] = R[key[j]];
+  __syncthreads();  // hist cleared
+  KT eici = 0, selfw = 0;
+  uint32_t slot[HUB_ITEMS];  // bucket << 16 | rank within the chunk's bucket
+#pragma unroll
+  for (int j = 0; j < HUB_ITEMS; ++j) {
+    if (key[j] == MGL_EMPTY) continue;
+    const vid u = key[j];
+    const vid cj = r[j].c;
+    const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
+    if (cand) {
+      const uint32_t bk = hub_bucket(cj, P);
+      slot[j] = (bk << 16) | atomicAdd(&hist[bk], 1u);
+      key[j] = cj;
+    } else {
+      key[j] = MGL_EMPTY;
+      if (cj == ci) {
+        const ST wv = wload<M>(A.w, e0 + (int64_t)j * HUB_T + threadIdx.x);
+        if (u != vp) eici += (KT)wv;
+        else selfw += (KT)wv;
+      }
+    }
+  }
+  __syncthreads();
+  // exclusive scan of the P <= 1024 bucket counts: four buckets per thread
+  uint32_t c4[4], s4 = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t b = 4 * threadIdx.x + k;
+    c4[k] = b < P ? hist[b] : 0u;
+    s4 += c4[k];
+  }
+  uint32_t inc = s4;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(inc, d, 64);
+    if (l >= d) inc += y;
+  }
+  if (l == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = inc - s4;
+  for (int k = 0; k < w; ++k) off += wsum[k];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t b = 4 * threadIdx.x + k;
+    if (b < P) hist[b] = off;
+    off += c4[k];
+  }
+  if (threadIdx.x == HUB_T - 1) hist[P] = off;  // candidates of the chunk
+  __syncthreads();
+  const int64_t cq = H.hub_cbo0[i] + (ch - H.hub_chunk0[i]) * (int64_t)(P + 1);
+  for (uint32_t b = threadIdx.x; b <= P; b += HUB_T) H.cbo[cq + b] = (int32_t)hist[b];
+#pragma unroll
+  for (int j = 0; j < HUB_ITEMS; ++j) {
+    if (key[j] == MGL_EMPTY) continue;
+    const int64_t dst = pos0 + hist[slot[j] >> 16] + (slot[j] & 0xFFFFu);
+    H.tmp_c[dst] = key[j];
+    ((AT*)H.tmp_a)[dst] = r[j].a;
+    if (M::HAS_W)
+      ((ST*)H.tmp_w)[dst] = wload<M>(A.w, e0 + (int64_t)j * HUB_T + threadIdx.x);
+  }
+  eici = bsum256(eici, red);
+  selfw = bsum256(selfw, red);
+  if (threadIdx.x == 0) {
+    if (eici != (KT)0) hatom_add(&((KT*)H.hub_eici)[i], eici);
+    if (selfw != (KT)0) hatom_add(&((KT*)H.hub_self)[i], selfw);
+  }
+}
+
 // ---- scatter ----------------------------------------------------------------
 template <class M, class AT>
 __global__ __launch_bounds__(HUB_T) void k_hub_scatter(MoveArgs A, HubPlan H) {
@@ -230,32 +346,58 @@ __device__ __forceinline__ int hub_ins_packed(uint64_t* T, vid key, ST w, uint32
   return -2;
 }
 
-// One workgroup per bucket. Buckets of at most IT entries per thread (and at
-// most 7/8 of the largest LDS table) use an owner-based LDS table as the move
-// kernels do: the creator of a key evaluates it with the total it staged, so
-// the table is neither scanned nor sized beyond the bucket. Larger buckets
-// (many entries of few communities) first try an LDS table that is scanned at
-// the end (4-byte sums); if their distinct communities overfill it, or the
-// sums take 8 bytes, an HBM table of 2*cnt slots at 2*s0 is used.
-template <class M, class AT, int IT, bool CNT>
-__global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hmax) {
+// Where the candidates of one bucket are. Contiguous: after the scatter,
+// positions s0 .. s0+cnt of the bucketed arrays. Segmented: one segment in the
+// region of every chunk of the hub (LDS prefix of the segment lengths).
+struct BucketContig {
+  const vid* c;
+  const void* a;
+  const void* w;
+  int64_t s0;
+  __device__ __forceinline__ int64_t at(int64_t x) const { return s0 + x; }
+};
+struct BucketSegs {
+  const vid* c;
+  const void* a;
+  const void* w;
+  int64_t base;          // tmp position of the hub's first chunk
+  const int32_t* pre;    // LDS [nch+1]: segment prefix
+  const int32_t* src;    // LDS [nch]: segment start relative to base
+  int nch;
+  __device__ __forceinline__ int64_t at(int64_t x) const {
+    int lo = 0, hi = nch;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if ((int64_t)pre[mid] <= x) lo = mid;
+      else hi = mid;
+    }
+    return base + (int64_t)src[lo] + (x - (int64_t)pre[lo]);
+  }
+};
+
+// Aggregation of one bucket of hub i by a workgroup of 1024 threads. Buckets
+// of at most IT entries per thread (and at most 7/8 of the largest LDS table)
+// use an owner-based LDS table as the move kernels do: the creator of a key
+// evaluates it with the total it staged, so the table is neither scanned nor
+// sized beyond the bucket. Larger buckets (many entries of few communities)
+// first try an LDS table that is scanned at the end (4-byte sums); if their
+// distinct communities overfill it, or the sums take 8 bytes, an HBM table of
+// 2*cnt slots from the pass's pool (H.galloc) is used.
+template <class M, class AT, int IT, bool CNT, class SRC>
+__device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubPlan& H,
+                                                   uint32_t Hmax, int64_t b, int64_t i, vid v,
+                                                   int g, const SRC& src, int64_t cnt,
+                                                   unsigned char* smem) {
   typedef typename M::S ST;
   typedef typename M::K KT;
   constexpr bool PACK = sizeof(ST) == 4;
   typedef OwnTab<ST, PACK> TB;
-  extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double rg[16];
   __shared__ vid rd[16];
   __shared__ int rc[16];
   __shared__ int ovf;
   __shared__ unsigned int ncre;
-  const int64_t b = blockIdx.x;
-  const int64_t i = H.bk_hub[b];
-  const vid v = H.hub_v[i];
-  const int g = A.gid ? A.gid[v] : 0;
-  if (!A.gs[g].active) return;
-  const int64_t s0 = H.boff[b], s1 = H.boff[b + 1];
-  const int64_t cnt = s1 - s0;
+  __shared__ unsigned long long gbase;
   const vid ci = A.c[v];
   const bool small = A.tie_small ? true : (A.up != 0);
   double best = 0.0;
@@ -269,7 +411,7 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
     const double Mh = A.gs[g].mass / 2.0;
     const double deici = (double)((const KT*)H.hub_eici)[i];
     const double dki = (double)ki, daci = (double)aci_k;
-    const AT* ba = (const AT*)H.bkt_a;
+    const AT* ba = (const AT*)src.a;
     if (cnt <= (int64_t)IT * 1024 && (uint64_t)cnt * 8 <= (uint64_t)Hmax * 7) {
       const uint32_t Ht = hub_table_size(cnt, Hmax);
       TB tb;
@@ -282,11 +424,13 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
       ST wv[IT];
 #pragma unroll
       for (int j = 0; j < IT; ++j) {
-        const int64_t x = s0 + threadIdx.x + (int64_t)j * 1024;
-        key[j] = x < s1 ? H.bkt_c[x] : MGL_EMPTY;
-        if (key[j] != MGL_EMPTY) {
-          oa[j] = ba[x];
-          wv[j] = M::HAS_W ? ((const ST*)H.bkt_w)[x] : (ST)1;
+        const int64_t x = threadIdx.x + (int64_t)j * 1024;
+        key[j] = MGL_EMPTY;
+        if (x < cnt) {
+          const int64_t q = src.at(x);
+          key[j] = src.c[q];
+          oa[j] = ba[q];
+          wv[j] = M::HAS_W ? ((const ST*)src.w)[q] : (ST)1;
         }
       }
 #pragma unroll
@@ -309,12 +453,13 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
         }
         __syncthreads();
         const unsigned lim = Hb / 8 * 7;
-        for (int64_t j = s0 + threadIdx.x; j < s1; j += 1024) {
+        for (int64_t x = threadIdx.x; x < cnt; x += 1024) {
           if (__hip_atomic_load(&ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
-          const ST w1 = M::HAS_W ? ((const ST*)H.bkt_w)[j] : (ST)1;
-          const int h = hub_ins_packed<ST>(T, H.bkt_c[j], w1, Hb, pc);
+          const int64_t q = src.at(x);
+          const ST w1 = M::HAS_W ? ((const ST*)src.w)[q] : (ST)1;
+          const int h = hub_ins_packed<ST>(T, src.c[q], w1, Hb, pc);
           if (h >= 0) {
-            Avs[h] = ba[j];
+            Avs[h] = ba[q];
             if (atomicAdd(&ncre, 1u) >= lim)
               __hip_atomic_store(&ovf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           } else if (h == -2) {
@@ -339,28 +484,31 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
         __syncthreads();  // the LDS table is not reused below, but ovf is read above
       }
       if (!done) {
-      const uint32_t Ht = (uint32_t)(2 * cnt);
-      vid* K = H.gk + 2 * s0;
-      ST* V = (ST*)H.gv + 2 * s0;
-      AT* Av = (AT*)H.ga + 2 * s0;
-      for (uint32_t s = threadIdx.x; s < Ht; s += 1024) { K[s] = MGL_EMPTY; V[s] = (ST)0; }
-      __threadfence();
-      __syncthreads();
-      for (int64_t j = s0 + threadIdx.x; j < s1; j += 1024) {
-        const ST w1 = M::HAS_W ? ((const ST*)H.bkt_w)[j] : (ST)1;
-        tab_insert<ST, AT, true>(K, V, Av, H.bkt_c[j], w1, ba[j], Ht);
-      }
-      __threadfence();
-      __syncthreads();
-      for (uint32_t s = threadIdx.x; s < Ht; s += 1024) {
-        const vid cj = __hip_atomic_load(&K[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (cj == MGL_EMPTY) continue;
-        const ST ev = __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const AT ak = __hip_atomic_load(&Av[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ++occ;
-        const double wij = gain((double)ev, deici, dki, daci, (double)ak, Mh);
-        if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
-      }
+        const uint32_t Ht = (uint32_t)(2 * cnt);
+        if (threadIdx.x == 0) gbase = atomicAdd(H.galloc, (unsigned long long)Ht);
+        __syncthreads();
+        vid* K = H.gk + gbase;
+        ST* V = (ST*)H.gv + gbase;
+        AT* Av = (AT*)H.ga + gbase;
+        for (uint32_t s = threadIdx.x; s < Ht; s += 1024) { K[s] = MGL_EMPTY; V[s] = (ST)0; }
+        __threadfence();
+        __syncthreads();
+        for (int64_t x = threadIdx.x; x < cnt; x += 1024) {
+          const int64_t q = src.at(x);
+          const ST w1 = M::HAS_W ? ((const ST*)src.w)[q] : (ST)1;
+          tab_insert<ST, AT, true>(K, V, Av, src.c[q], w1, ba[q], Ht);
+        }
+        __threadfence();
+        __syncthreads();
+        for (uint32_t s = threadIdx.x; s < Ht; s += 1024) {
+          const vid cj = __hip_atomic_load(&K[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          if (cj == MGL_EMPTY) continue;
+          const ST ev = __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          const AT ak = __hip_atomic_load(&Av[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          ++occ;
+          const double wij = gain((double)ev, deici, dki, daci, (double)ak, Mh);
+          if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+        }
       }
     }
   }
@@ -381,6 +529,79 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
     H.bdst[b] = dst;
     if (CNT && A.stats && tot) atomicAdd(A.stats, (unsigned long long)tot);
   }
+}
+
+// One workgroup per bucket, after the scatter (contiguous buckets).
+template <class M, class AT, int IT, bool CNT>
+__global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hmax) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int64_t b = blockIdx.x;
+  const int64_t i = H.bk_hub[b];
+  const vid v = H.hub_v[i];
+  const int g = A.gid ? A.gid[v] : 0;
+  if (!A.gs[g].active) return;
+  const int64_t s0 = H.boff[b], s1 = H.boff[b + 1];
+  BucketContig src{H.bkt_c, H.bkt_a, H.bkt_w, s0};
+  hub_aggregate_body<M, AT, IT, CNT>(A, H, Hmax, b, i, v, g, src, s1 - s0, smem);
+}
+
+// Block-wide exclusive scan of one value per thread (1024 threads).
+__device__ __forceinline__ int32_t exscan1024(int32_t x, int32_t* wsum /* [16] */) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int32_t inc = x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int32_t y = __shfl_up(inc, d, 64);
+    if (l >= d) inc += y;
+  }
+  if (l == 63) wsum[w] = inc;
+  __syncthreads();
+  int32_t off = 0;
+  for (int k = 0; k < w; ++k) off += wsum[k];
+  __syncthreads();
+  return off + inc - x;
+}
+
+// One workgroup per bucket in the segmented form: the bucket's segment in each
+// chunk of its hub is located from the chunks' bucket offsets.
+template <class M, class AT, int IT, bool CNT>
+__global__ __launch_bounds__(1024) void k_hub_aggregate_seg(MoveArgs A, HubPlan H, uint32_t Hmax) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int32_t pre[HUB_SEG_MAX + 1];
+  __shared__ int32_t srcs[HUB_SEG_MAX];
+  __shared__ int32_t wsum[16];
+  const int64_t b = blockIdx.x;
+  const int64_t i = H.bk_hub[b];
+  const vid v = H.hub_v[i];
+  const int g = A.gid ? A.gid[v] : 0;
+  if (!A.gs[g].active) return;
+  const int64_t lb = b - H.hub_bk[i];
+  const int64_t P = H.hub_bk[i + 1] - H.hub_bk[i];
+  const int64_t c0 = H.hub_chunk0[i];
+  const int nch = (int)(H.hub_chunk0[i + 1] - c0);
+  const int64_t cb0 = H.hub_cbo0[i];
+  const int64_t base = H.chunk[3 * c0 + 2];
+  // segment lengths (two chunks per thread: nch <= HUB_SEG_MAX = 2 x 1024)
+  int32_t len[2];
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    const int c = 2 * threadIdx.x + k;
+    len[k] = 0;
+    if (c < nch) {
+      const int64_t q = cb0 + (int64_t)c * (P + 1) + lb;
+      const int32_t s = H.cbo[q];
+      len[k] = H.cbo[q + 1] - s;
+      srcs[c] = (int32_t)(H.chunk[3 * (c0 + c) + 2] - base) + s;
+    }
+  }
+  const int32_t ex = exscan1024(len[0] + len[1], wsum);
+  if (2 * threadIdx.x < nch) pre[2 * threadIdx.x] = ex;
+  if (2 * threadIdx.x + 1 < nch) pre[2 * threadIdx.x + 1] = ex + len[0];
+  if (threadIdx.x == 1023) pre[nch] = ex + len[0] + len[1];
+  __syncthreads();
+  const int64_t cnt = pre[nch];
+  BucketSegs src{H.tmp_c, H.tmp_a, H.tmp_w, base, pre, srcs, nch};
+  hub_aggregate_body<M, AT, IT, CNT>(A, H, Hmax, b, i, v, g, src, cnt, smem);
 }
 
 // ---- finalize -----------------------------------------------------------------
@@ -412,23 +633,31 @@ template <class M, class AT>
 static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t st) {
   typedef typename M::S ST;
   if (H.m <= 0) return 0;
-  launch_fill_bytes0(H.bcount, 8 * (size_t)H.nbuckets, st);
-  launch_fill_bytes0(H.hub_eici, 8 * (size_t)H.m, st);
-  launch_fill_bytes0(H.hub_self, 8 * (size_t)H.m, st);
-  k_hub_count<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
-  scan_exclusive_i64(H.bcount, H.boff, H.nbuckets, H.scan_tmp, st);
-  {
-    int64_t gr = (H.nbuckets + 255) / 256;
-    if (gr > 4096) gr = 4096;
-    k_hub_copy<<<(unsigned)gr, 256, 0, st>>>(H.boff, H.bcur, H.nbuckets);
-  }
-  k_hub_scatter<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+  const bool cnt = A.stats || A.probes;
   const size_t lds = (sizeof(ST) == 4) ? 8 * (size_t)Hlds
                                        : ((4 * (size_t)Hlds + 15) / 16) * 16 + sizeof(ST) * (size_t)Hlds;
-  if (A.stats || A.probes)
-    k_hub_aggregate<M, AT, HUB_AGG_IT, true><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
-  else
-    k_hub_aggregate<M, AT, HUB_AGG_IT, false><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+  k_hub_head<<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(H);
+  if (H.segmented) {
+    k_hub_count_seg<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+    if (cnt)
+      k_hub_aggregate_seg<M, AT, HUB_AGG_IT, true><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+    else
+      k_hub_aggregate_seg<M, AT, HUB_AGG_IT, false><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+  } else {
+    launch_fill_bytes0(H.bcount, 8 * (size_t)H.nbuckets, st);
+    k_hub_count<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+    scan_exclusive_i64(H.bcount, H.boff, H.nbuckets, H.scan_tmp, st);
+    {
+      int64_t gr = (H.nbuckets + 255) / 256;
+      if (gr > 4096) gr = 4096;
+      k_hub_copy<<<(unsigned)gr, 256, 0, st>>>(H.boff, H.bcur, H.nbuckets);
+    }
+    k_hub_scatter<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+    if (cnt)
+      k_hub_aggregate<M, AT, HUB_AGG_IT, true><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+    else
+      k_hub_aggregate<M, AT, HUB_AGG_IT, false><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+  }
   k_hub_final<M, AT><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
   return 0;
 }
@@ -449,11 +678,12 @@ int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStre
 }
 
 void hub_allow_lds(int bytes) {
-#define A_(M_, AT_)                                                                   \
-  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_, HUB_AGG_IT, false>,     \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);         \
-  (void)hipFuncSetAttribute((const void*)k_hub_aggregate<M_, AT_, HUB_AGG_IT, true>,      \
+#define B_(K_, M_, AT_, C_)                                                  \
+  (void)hipFuncSetAttribute((const void*)K_<M_, AT_, HUB_AGG_IT, C_>,        \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+#define A_(M_, AT_)                                                                   \
+  B_(k_hub_aggregate, M_, AT_, false) B_(k_hub_aggregate, M_, AT_, true)              \
+  B_(k_hub_aggregate_seg, M_, AT_, false) B_(k_hub_aggregate_seg, M_, AT_, true)
   A_(ModeUnit, int32_t) A_(ModeUnit, int64_t) A_(ModeInt, int64_t) A_(ModeF64, double)
   A_(ModeF32, double) A_(ModeI32, int32_t) A_(ModeI32, int64_t)
 #undef A_
@@ -461,3 +691,4 @@ void hub_allow_lds(int bytes) {
 
 int hub_chunk_edges() { return HUB_CHUNK; }
 int hub_max_buckets() { return HUB_MAXP; }
+int hub_seg_max() { return HUB_SEG_MAX; }

@@ -107,11 +107,21 @@ struct HubPlan {
   void* ga = nullptr;
   void* scan_tmp = nullptr;
   uint32_t lds_slots = 0;           // LDS table slots of the aggregate kernel
+  // Segmented form (every hub of at most hub_seg_max() chunks): each chunk's
+  // candidates stay in its region of tmp, grouped by bucket, and cbo holds the
+  // chunk's P_i + 1 bucket offsets; the aggregate reads a bucket's segments
+  // from every chunk of its hub, so no scatter pass and no global scan.
+  int segmented = 0;
+  const int64_t* hub_chunk0 = nullptr;  // [m+1] first chunk of each hub
+  const int64_t* hub_cbo0 = nullptr;    // [m] first cbo entry of each hub
+  int32_t* cbo = nullptr;               // [sum over hubs of chunks x (P_i + 1)]
+  unsigned long long* galloc = nullptr; // [1] HBM-table slots handed out in this pass
 };
 int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStream_t st);
 void hub_allow_lds(int bytes);
 int hub_chunk_edges();
 int hub_max_buckets();
+int hub_seg_max();  // chunks per hub of the segmented form
 void launch_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out, hipStream_t st);
 
 // ---- per-vertex row reductions (vertex weights, modularity sum_in) ----------

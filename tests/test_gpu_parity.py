@@ -124,13 +124,15 @@ def hub_graph(mgl):
     return mgl.gen_rmat(17, 48, seed=13)
 
 
+@pytest.mark.parametrize("segmented", [1, 0])
 @pytest.mark.parametrize("bucket", [4096, 1 << 20])
-def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, bucket):
+def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, bucket, segmented):
     # bucket = 2^20 makes each hub a single bucket larger than the LDS table,
-    # which exercises the HBM-table fallback of the aggregate step
+    # which exercises the HBM-table fallback of the aggregate step;
+    # segmented = 0 forces the scatter form of the hub path
     g = hub_graph(mgl)
     tf = tmp_path / "tune.txt"
-    tf.write_text(f"hub_bucket {bucket}\n" + "".join(
+    tf.write_text(f"hub_bucket {bucket}\nhub_segmented {segmented}\n" + "".join(
         f"{u} {sh} {s_} {sz}\n" for u, sh, s_, sz in
         [(0, 0, 0, 0), (4, 1, 4, 3), (8, 1, 8, 4), (16, 1, 16, 5), (32, 1, 32, 6),
          (64, 1, 64, 7), (128, 1, 64, 8), (256, 1, 64, 9), (1024, 2, 256, 2048),
@@ -150,6 +152,21 @@ def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, bucket):
         for up in (True, False):
             np.testing.assert_array_equal(
                 s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
+    s.close()
+
+
+def test_tuning_file_applies_to_its_call_only(ctx, mgl, tmp_path):
+    g = mgl.gen_rmat(12, 16, seed=3)
+    s = ctx.session(g)
+    before = s.bins()
+    s.close()
+    tf = tmp_path / "tune.txt"
+    tf.write_text("hub_bucket 64\n0 0 0 0\n32 1 32 6\n-1 3 0 0\n")
+    s = ctx.session(g, tuning_file=str(tf))
+    assert len(s.bins()) == 3
+    s.close()
+    s = ctx.session(g)
+    assert s.bins() == before
     s.close()
 
 
