@@ -152,6 +152,9 @@ typedef struct {
   /* LDS hash tables of the move pass: key inserts, compare-and-swap probes, and
    * inserts that needed more than one probe (first-probe hit rate = 1 - multi/inserts) */
   int64_t lds_inserts, lds_probes, lds_multi_probe;
+  /* last pass: duration of each bin's move launch (the hub bin: its whole path), 0 if empty */
+  int n_bins;
+  double bin_ms[16];
 } mgl_pass_stats;
 
 mgl_status mgl_session_create(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* o,

@@ -75,11 +75,13 @@ class PassStats(C.Structure):
                 ("q", C.c_double), ("dominant_kernel", C.c_char * 96), ("hub_ms", C.c_double),
                 ("hub_vertices", C.c_int64), ("hub_entries", C.c_int64),
                 ("hub_distinct", C.c_int64), ("lds_inserts", C.c_int64),
-                ("lds_probes", C.c_int64), ("lds_multi_probe", C.c_int64)]
+                ("lds_probes", C.c_int64), ("lds_multi_probe", C.c_int64),
+                ("n_bins", C.c_int), ("bin_ms", C.c_double * 16)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
         d["dominant_kernel"] = self.dominant_kernel.decode()
+        d["bin_ms"] = list(self.bin_ms)[:self.n_bins]
         return d
 
 

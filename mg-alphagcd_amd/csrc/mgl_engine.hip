@@ -1156,9 +1156,9 @@ static void parallel_for(int64_t n, F f) {
   for (auto& x : th) x.join();
 }
 
-// Smallest S >= 0 such that every weight * 2^S is an integer; -1 if a weight is
-// not finite and positive or the scaled total would exceed 2^62.
-static int dyadic_scale(const float* w, int64_t nnz) {
+// Smallest S >= 0 such that every weight * 2^S is an integer, and the sum of
+// the weights in *total; -1 if a weight is not finite and positive.
+static int dyadic_scale(const float* w, int64_t nnz, long double* total) {
   int nt = 16;
   std::vector<int> sc(nt, 0);
   std::vector<int> bad(nt, 0);
@@ -1186,7 +1186,7 @@ static int dyadic_scale(const float* w, int64_t nnz) {
     S = std::max(S, sc[t]);
     T += tot[t];
   }
-  if (T * std::ldexp(1.0L, S) > std::ldexp(1.0L, 62)) return -1;
+  *total = T;
   return S;
 }
 
@@ -1277,15 +1277,24 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
   prec = o.precision;
   int S = 0;
   if (weighted && (prec == MGL_PREC_AUTO || prec == MGL_PREC_EXACT)) {
+    // one scale 2^S for the batch; every graph's scaled total (unit weights
+    // count 2^S each) must stay below 2^62
+    std::vector<long double> tot(G, 0.0L);
     for (int i = 0; i < G && S >= 0; ++i) {
-      if (!graphs[i].w) continue;
-      const int si = dyadic_scale(graphs[i].w, graphs[i].nnz);
+      if (!graphs[i].w) {
+        tot[i] = (long double)graphs[i].nnz;
+        continue;
+      }
+      const int si = dyadic_scale(graphs[i].w, graphs[i].nnz, &tot[i]);
       S = si < 0 ? -1 : std::max(S, si);
     }
+    for (int i = 0; i < G && S >= 0; ++i)
+      if (tot[i] * std::ldexp(1.0L, S) > std::ldexp(1.0L, 62)) S = -1;
     if (S < 0) {
       if (prec == MGL_PREC_EXACT)
         FAIL(ctx, MGL_UNSUPPORTED,
-             "EXACT precision needs positive weights that are integer multiples of 2^-S");
+             "EXACT precision needs positive weights that are integer multiples of 2^-S "
+             "with every graph's total below 2^(62-S)");
       prec = MGL_PREC_FP64;
       S = 0;
     }
@@ -1314,7 +1323,7 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
     GState& s = hgs[i];
     if (prec == MGL_PREC_EXACT) {
       if (!gr.w) {
-        s.mass_i = gr.nnz;
+        s.mass_i = gr.nnz * ((int64_t)1 << S);  // unit weights carry 2^S in a weighted batch
       } else {
         std::vector<int64_t> part(16, 0);
         const int sc = S;
@@ -1711,6 +1720,8 @@ mgl_status mgl_session_reset(mgl_session* s, double* q0) {
   return MGL_OK;
 }
 
+static_assert(MGL_MAX_BINS <= 16, "mgl_pass_stats.bin_ms holds 16 bins");
+
 mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   if (!s) return MGL_INVALID_ARG;
   mgl_ctx* ctx = s->ctx;
@@ -1767,9 +1778,11 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   if (s->count_it == s->it) memcpy(dh, s->cnt, sizeof(dh));
   // the dominant launch among bins served by one kernel; the hub path apart
   int hub_bin = -1;
+  st.n_bins = E.L.nb;
   for (int b = E.L.nb - 1; b >= 1; --b) {
     if (E.bin_n[b] == 0) continue;
     HIPC(ctx, hipEventElapsedTime(&ms, ev[2 * b], ev[2 * b + 1]));
+    st.bin_ms[b] = ms;
     if (E.L.b[b].shape == SHAPE_GLOBAL) {
       st.hub_ms = ms;
       hub_bin = b;

@@ -223,6 +223,22 @@ def test_batched_equals_plain(ctx, mgl, graph):
         np.testing.assert_array_equal(b.final_assign, p.final_assign)
 
 
+def test_batched_weighted_and_unit_graphs(ctx, mgl):
+    # unit-weight graphs inside a batch with FP32 weights run in the batch's
+    # fixed-point scale (each unit weight is 2^S); their 2m must be scaled too
+    graphs = [mgl.gen_rmat(11, 24, seed=900, weights=1), mgl.gen_rmat(12, 25, seed=901),
+              mgl.gen_sbm(3000, 12, min_size=64, seed=902, weights=0, deg_in=16, deg_out=3),
+              mgl.gen_sbm(2000, 8, min_size=64, seed=903, weights=1, deg_in=16, deg_out=3)]
+    batch = ctx.louvain_batched(graphs, keep_levels=True)
+    for g, b in zip(graphs, batch):
+        p = ctx.louvain(g, keep_levels=True)
+        assert len(b.levels) == len(p.levels)
+        for lb, lp in zip(b.levels, p.levels):
+            assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
+            np.testing.assert_array_equal(lb.assign, lp.assign)
+        np.testing.assert_array_equal(b.final_assign, p.final_assign)
+
+
 def test_repeat_runs_identical(ctx, mgl):
     g = mgl.gen_rmat(15, 16, seed=21)
     a = ctx.louvain(g, keep_levels=True)
