@@ -181,21 +181,40 @@ def traffic_for(workload, kernel):
 
 
 def timed_passes(sess, steps, warmup):
-    for i in range(warmup):
-        q, acc = sess.pass_()
-        log(f"warmup pass {i + 1}: Q={q:.9f} accepted={acc} {sess.stats()['pass_ms']:.3f} ms")
-    # each step: the counters of the pass's move first (untimed, state
-    # unchanged), then the pass itself on the wall clock; pass_() returns
-    # after a stream synchronisation, so the sum of the step walls is the
-    # device time of the K passes plus their host overhead
+    """W warm-up passes from the session's reset state, then K passes timed as
+    one block: the wall clock runs between two points where the device is idle
+    (every pass_() ends with a stream synchronisation, as the reference's host
+    reads Q after each pass). The counters of each pass's move (distinct
+    (vertex, community) pairs, LDS probes; global atomics, so never inside a
+    timed pass) come from a replay: reset, the same warm-up, then count() before
+    each pass. The replay follows the same trajectory (checked on Q; in the
+    exact modes it is bit-identical)."""
+    def warm():
+        for i in range(warmup):
+            q, acc = sess.pass_()
+            log(f"warmup pass {i + 1}: Q={q:.9f} accepted={acc} "
+                f"{sess.stats()['pass_ms']:.3f} ms")
+    warm()
     stats = []
-    wall = 0.0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        sess.pass_()
+        stats.append(sess.stats())
+    wall = time.perf_counter() - t0
+    # replay with the counters
+    sess.reset()
+    warm()
+    keys = ("distinct", "dominant_distinct", "hub_distinct", "lds_inserts", "lds_probes",
+            "lds_multi_probe")
     for i in range(steps):
         sess.count()
-        t0 = time.perf_counter()
         sess.pass_()
-        wall += time.perf_counter() - t0
-        stats.append(sess.stats())
+        c = sess.stats()
+        if c["q"] != stats[i]["q"]:
+            log(f"replay pass {warmup + i + 1}: Q {c['q']!r} differs from the timed "
+                f"{stats[i]['q']!r}")
+        for k in keys:
+            stats[i][k] = c[k]
     for i, s in enumerate(stats):
         log(f"pass {warmup + i + 1}: pass {s['pass_ms']:.3f} ms move {s['move_ms']:.3f} "
             f"update {s['update_ms']:.3f} mod {s['modularity_ms']:.3f} | dominant "
@@ -203,6 +222,7 @@ def timed_passes(sess, steps, warmup):
             f"({s['dominant_vertices']} v, {s['dominant_entries']} e) | hub path "
             f"{s['hub_ms']:.3f} ms ({s['hub_vertices']} v, {s['hub_entries']} e) | moved "
             f"{s['moved']} distinct {s['distinct']} Q={s['q']:.9f}")
+        log("   bin ms: " + " ".join(f"{x:.3f}" for x in s["bin_ms"]))
     return stats, wall
 
 

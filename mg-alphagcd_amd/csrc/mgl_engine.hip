@@ -35,8 +35,12 @@
 struct Tuning {
   BinLayout bins;
   bool have_bins = false;     // false: built-in layout for the device
-  int64_t hub_bucket = 4096;  // hub candidates per bucket on average
+  int64_t hub_bucket = 3072;  // hub candidates per bucket on average (capped by the tables)
+  bool hub_bucket_clamp = true; // false: hub_bucket as given (oversized buckets take the
+                                // scanned-LDS or HBM-table fallback; used by tests)
   int hub_segmented = 1;      // 0: the hub path always scatters into buckets
+  int hub_agg_threads = 512;  // threads per hub-aggregate workgroup (256, 512, 1024)
+  int hub_agg_per_cu = 4;     // hub-aggregate workgroups sharing a CU's LDS (1..8)
 };
 
 struct mgl_ctx {
@@ -381,7 +385,23 @@ struct Engine {
       HIPC(ctx, hipMemcpyAsync(ranges.data(), rr.p, 16 * m, hipMemcpyDeviceToHost, st()));
       HIPC(ctx, hipStreamSynchronize(st()));
       const int64_t CH = hub_chunk_edges();
-      const int64_t BK = tune.hub_bucket;
+      // owner tables of the aggregate: (key, sum) slots in a share of the
+      // CU's LDS, so that hub_agg_per_cu buckets share a CU; prime-sized. The
+      // segmented aggregate also holds its hub's segment table (two 4-byte
+      // entries per chunk) in static LDS.
+      const int asb = slot_bytes(g.mode) == 4 ? 8 : 4 + slot_bytes(g.mode);
+      const int seg = 8 * hub_seg_max() + 64;
+      const uint32_t Hlds = (uint32_t)prime_at_most(
+          (int)((ctx->lds_max / tune.hub_agg_per_cu - 512 - seg) / asb));
+      // a bucket's candidates must fit the owner path (at most 7/8 of the
+      // table and hub_agg_cap() per workgroup) with a quarter to spare for the
+      // spread of the hash partition
+      int64_t BK = tune.hub_bucket;
+      if (tune.hub_bucket_clamp) {
+        BK = std::min<int64_t>(BK, (int64_t)Hlds * 7 / 8 * 4 / 5);
+        BK = std::min<int64_t>(BK, (int64_t)hub_agg_cap() * 4 / 5);
+        if (BK < 64) BK = 64;
+      }
       std::vector<int64_t> chunk, hbkv(m + 1, 0), bkhub, chunk0(m + 1, 0), cbo0(m, 0);
       int64_t pos = 0, ncbo = 0, max_nch = 0;
       for (int64_t i = 0; i < m; ++i) {
@@ -473,15 +493,8 @@ struct Engine {
       hub.hub_cbo0 = hcbo0.as<int64_t>();
       hub.cbo = hcbo.as<int32_t>();
       hub.galloc = hgalloc.as<unsigned long long>();
-      // owner tables of the aggregate: (key, sum) slots in half of the
-      // workgroup LDS, so that two buckets share a CU; prime-sized
-      {
-        const int sb = slot_bytes(g.mode) == 4 ? 8 : 4 + slot_bytes(g.mode);
-        // the segmented aggregate also holds its hub's segment table (two
-        // 4-byte entries per chunk) in static LDS
-        const int seg = hub.segmented ? 8 * hub_seg_max() + 64 : 0;
-        hub.lds_slots = (uint32_t)prime_at_most((int)((ctx->lds_max / 2 - 1024 - seg) / sb));
-      }
+      hub.lds_slots = Hlds;
+      hub.agg_bs = tune.hub_agg_threads;
     }
     return MGL_OK;
   }
@@ -2051,9 +2064,26 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
       tu.hub_bucket = hb;
       continue;
     }
+    if (sscanf(line, "hub_bucket_unclamped %lld", &hb) == 1) {
+      if (hb < 1) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_bucket", path);
+      tu.hub_bucket = hb;
+      tu.hub_bucket_clamp = false;
+      continue;
+    }
     int hs;
     if (sscanf(line, "hub_segmented %d", &hs) == 1) {
       tu.hub_segmented = hs != 0;
+      continue;
+    }
+    if (sscanf(line, "hub_agg_threads %d", &hs) == 1) {
+      if (hs != 256 && hs != 512 && hs != 1024)
+        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_agg_threads", path);
+      tu.hub_agg_threads = hs;
+      continue;
+    }
+    if (sscanf(line, "hub_agg_per_cu %d", &hs) == 1) {
+      if (hs < 1 || hs > 8) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_agg_per_cu", path);
+      tu.hub_agg_per_cu = hs;
       continue;
     }
     long long up;

@@ -25,8 +25,10 @@
 #define HUB_T 256
 #define HUB_ITEMS (HUB_CHUNK / HUB_T)
 #define HUB_MAXP 1024
-#define HUB_AGG_IT 8
-#define HUB_SEG_MAX 2048  // chunks per hub in the segmented form
+#define HUB_SEG_MAX 512   // chunks per hub in the segmented form
+// Aggregate workgroups: BS threads (HubPlan::agg_bs: 256, 512 or 1024),
+// 6,144 / BS candidates per thread on the owner path.
+#define HUB_AGG_CAP 6144
 
 __device__ __forceinline__ uint32_t hub_bucket(vid key, uint32_t P) {
   return (uint32_t)(((uint64_t)((uint32_t)key * 2654435761u) * P) >> 32);
@@ -375,7 +377,7 @@ struct BucketSegs {
   }
 };
 
-// Aggregation of one bucket of hub i by a workgroup of 1024 threads. Buckets
+// Aggregation of one bucket of hub i by a workgroup of BS threads. Buckets
 // of at most IT entries per thread (and at most 7/8 of the largest LDS table)
 // use an owner-based LDS table as the move kernels do: the creator of a key
 // evaluates it with the total it staged, so the table is neither scanned nor
@@ -383,7 +385,7 @@ struct BucketSegs {
 // first try an LDS table that is scanned at the end (4-byte sums); if their
 // distinct communities overfill it, or the sums take 8 bytes, an HBM table of
 // 2*cnt slots from the pass's pool (H.galloc) is used.
-template <class M, class AT, int IT, bool CNT, class SRC>
+template <class M, class AT, int BS, int IT, bool CNT, class SRC>
 __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubPlan& H,
                                                    uint32_t Hmax, int64_t b, int64_t i, vid v,
                                                    int g, const SRC& src, int64_t cnt,
@@ -392,9 +394,9 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
   typedef typename M::K KT;
   constexpr bool PACK = sizeof(ST) == 4;
   typedef OwnTab<ST, PACK> TB;
-  __shared__ double rg[16];
-  __shared__ vid rd[16];
-  __shared__ int rc[16];
+  __shared__ double rg[BS / 64];
+  __shared__ vid rd[BS / 64];
+  __shared__ int rc[BS / 64];
   __shared__ int ovf;
   __shared__ unsigned int ncre;
   __shared__ unsigned long long gbase;
@@ -412,11 +414,11 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
     const double deici = (double)((const KT*)H.hub_eici)[i];
     const double dki = (double)ki, daci = (double)aci_k;
     const AT* ba = (const AT*)src.a;
-    if (cnt <= (int64_t)IT * 1024 && (uint64_t)cnt * 8 <= (uint64_t)Hmax * 7) {
+    if (cnt <= (int64_t)IT * BS && (uint64_t)cnt * 8 <= (uint64_t)Hmax * 7) {
       const uint32_t Ht = hub_table_size(cnt, Hmax);
       TB tb;
       tb.init(smem, Ht);
-      for (uint32_t s = threadIdx.x; s < Ht; s += 1024) tb.clear(s);
+      for (uint32_t s = threadIdx.x; s < Ht; s += BS) tb.clear(s);
       __syncthreads();
       int own[IT];
       AT oa[IT];
@@ -424,7 +426,7 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
       ST wv[IT];
 #pragma unroll
       for (int j = 0; j < IT; ++j) {
-        const int64_t x = threadIdx.x + (int64_t)j * 1024;
+        const int64_t x = threadIdx.x + (int64_t)j * BS;
         key[j] = MGL_EMPTY;
         if (x < cnt) {
           const int64_t q = src.at(x);
@@ -446,14 +448,14 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
         uint64_t* T = (uint64_t*)smem;
         AT* Avs = (AT*)(smem + 8 * (size_t)Hb);
         const uint64_t empty = (uint64_t)(uint32_t)MGL_EMPTY << 32;
-        for (uint32_t s = threadIdx.x; s < Hb; s += 1024) T[s] = empty;
+        for (uint32_t s = threadIdx.x; s < Hb; s += BS) T[s] = empty;
         if (threadIdx.x == 0) {
           ovf = 0;
           ncre = 0;
         }
         __syncthreads();
         const unsigned lim = Hb / 8 * 7;
-        for (int64_t x = threadIdx.x; x < cnt; x += 1024) {
+        for (int64_t x = threadIdx.x; x < cnt; x += BS) {
           if (__hip_atomic_load(&ovf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) break;
           const int64_t q = src.at(x);
           const ST w1 = M::HAS_W ? ((const ST*)src.w)[q] : (ST)1;
@@ -468,7 +470,7 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
         }
         __syncthreads();
         if (!ovf) {
-          for (uint32_t s = threadIdx.x; s < Hb; s += 1024) {
+          for (uint32_t s = threadIdx.x; s < Hb; s += BS) {
             const uint64_t t = T[s];
             const vid cj = (vid)(uint32_t)(t >> 32);
             if (cj == MGL_EMPTY) continue;
@@ -490,17 +492,17 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
         vid* K = H.gk + gbase;
         ST* V = (ST*)H.gv + gbase;
         AT* Av = (AT*)H.ga + gbase;
-        for (uint32_t s = threadIdx.x; s < Ht; s += 1024) { K[s] = MGL_EMPTY; V[s] = (ST)0; }
+        for (uint32_t s = threadIdx.x; s < Ht; s += BS) { K[s] = MGL_EMPTY; V[s] = (ST)0; }
         __threadfence();
         __syncthreads();
-        for (int64_t x = threadIdx.x; x < cnt; x += 1024) {
+        for (int64_t x = threadIdx.x; x < cnt; x += BS) {
           const int64_t q = src.at(x);
           const ST w1 = M::HAS_W ? ((const ST*)src.w)[q] : (ST)1;
           tab_insert<ST, AT, true>(K, V, Av, src.c[q], w1, ba[q], Ht);
         }
         __threadfence();
         __syncthreads();
-        for (uint32_t s = threadIdx.x; s < Ht; s += 1024) {
+        for (uint32_t s = threadIdx.x; s < Ht; s += BS) {
           const vid cj = __hip_atomic_load(&K[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (cj == MGL_EMPTY) continue;
           const ST ev = __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -521,7 +523,7 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
   __syncthreads();
   if (threadIdx.x == 0) {
     int tot = 0;
-    for (int k2 = 0; k2 < 16; ++k2) {
+    for (int k2 = 0; k2 < BS / 64; ++k2) {
       if (better(rg[k2], rd[k2], best, dst, small)) { best = rg[k2]; dst = rd[k2]; }
       tot += rc[k2];
     }
@@ -532,8 +534,8 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
 }
 
 // One workgroup per bucket, after the scatter (contiguous buckets).
-template <class M, class AT, int IT, bool CNT>
-__global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hmax) {
+template <class M, class AT, int BS, int IT, bool CNT>
+__global__ __launch_bounds__(BS) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hmax) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int64_t b = blockIdx.x;
   const int64_t i = H.bk_hub[b];
@@ -542,11 +544,12 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate(MoveArgs A, HubPlan H, u
   if (!A.gs[g].active) return;
   const int64_t s0 = H.boff[b], s1 = H.boff[b + 1];
   BucketContig src{H.bkt_c, H.bkt_a, H.bkt_w, s0};
-  hub_aggregate_body<M, AT, IT, CNT>(A, H, Hmax, b, i, v, g, src, s1 - s0, smem);
+  hub_aggregate_body<M, AT, BS, IT, CNT>(A, H, Hmax, b, i, v, g, src, s1 - s0, smem);
 }
 
-// Block-wide exclusive scan of one value per thread (1024 threads).
-__device__ __forceinline__ int32_t exscan1024(int32_t x, int32_t* wsum /* [16] */) {
+// Block-wide exclusive scan of one value per thread (BS threads).
+template <int BS>
+__device__ __forceinline__ int32_t exscan_block(int32_t x, int32_t* wsum /* [BS / 64] */) {
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   int32_t inc = x;
 #pragma unroll
@@ -564,12 +567,12 @@ __device__ __forceinline__ int32_t exscan1024(int32_t x, int32_t* wsum /* [16] *
 
 // One workgroup per bucket in the segmented form: the bucket's segment in each
 // chunk of its hub is located from the chunks' bucket offsets.
-template <class M, class AT, int IT, bool CNT>
-__global__ __launch_bounds__(1024) void k_hub_aggregate_seg(MoveArgs A, HubPlan H, uint32_t Hmax) {
+template <class M, class AT, int BS, int IT, bool CNT>
+__global__ __launch_bounds__(BS) void k_hub_aggregate_seg(MoveArgs A, HubPlan H, uint32_t Hmax) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t pre[HUB_SEG_MAX + 1];
   __shared__ int32_t srcs[HUB_SEG_MAX];
-  __shared__ int32_t wsum[16];
+  __shared__ int32_t wsum[BS / 64];
   const int64_t b = blockIdx.x;
   const int64_t i = H.bk_hub[b];
   const vid v = H.hub_v[i];
@@ -581,11 +584,13 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate_seg(MoveArgs A, HubPlan 
   const int nch = (int)(H.hub_chunk0[i + 1] - c0);
   const int64_t cb0 = H.hub_cbo0[i];
   const int64_t base = H.chunk[3 * c0 + 2];
-  // segment lengths (two chunks per thread: nch <= HUB_SEG_MAX = 2 x 1024)
-  int32_t len[2];
+  // segment lengths, CPT consecutive chunks per thread (nch <= HUB_SEG_MAX)
+  constexpr int CPT = (HUB_SEG_MAX + BS - 1) / BS;
+  int32_t len[CPT];
+  int32_t tot = 0;
 #pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    const int c = 2 * threadIdx.x + k;
+  for (int k = 0; k < CPT; ++k) {
+    const int c = CPT * threadIdx.x + k;
     len[k] = 0;
     if (c < nch) {
       const int64_t q = cb0 + (int64_t)c * (P + 1) + lb;
@@ -593,15 +598,20 @@ __global__ __launch_bounds__(1024) void k_hub_aggregate_seg(MoveArgs A, HubPlan 
       len[k] = H.cbo[q + 1] - s;
       srcs[c] = (int32_t)(H.chunk[3 * (c0 + c) + 2] - base) + s;
     }
+    tot += len[k];
   }
-  const int32_t ex = exscan1024(len[0] + len[1], wsum);
-  if (2 * threadIdx.x < nch) pre[2 * threadIdx.x] = ex;
-  if (2 * threadIdx.x + 1 < nch) pre[2 * threadIdx.x + 1] = ex + len[0];
-  if (threadIdx.x == 1023) pre[nch] = ex + len[0] + len[1];
+  int32_t ex = exscan_block<BS>(tot, wsum);
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int c = CPT * threadIdx.x + k;
+    if (c < nch) pre[c] = ex;
+    ex += len[k];
+  }
+  if (threadIdx.x == BS - 1) pre[nch] = ex;
   __syncthreads();
   const int64_t cnt = pre[nch];
   BucketSegs src{H.tmp_c, H.tmp_a, H.tmp_w, base, pre, srcs, nch};
-  hub_aggregate_body<M, AT, IT, CNT>(A, H, Hmax, b, i, v, g, src, cnt, smem);
+  hub_aggregate_body<M, AT, BS, IT, CNT>(A, H, Hmax, b, i, v, g, src, cnt, smem);
 }
 
 // ---- finalize -----------------------------------------------------------------
@@ -637,12 +647,15 @@ static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t
   const size_t lds = (sizeof(ST) == 4) ? 8 * (size_t)Hlds
                                        : ((4 * (size_t)Hlds + 15) / 16) * 16 + sizeof(ST) * (size_t)Hlds;
   k_hub_head<<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(H);
+  const unsigned nbk = (unsigned)H.nbuckets;
+#define AGG_(K_, BS_)                                                                    \
+  if (H.agg_bs == BS_) {                                                                 \
+    if (cnt) K_<M, AT, BS_, HUB_AGG_CAP / BS_, true><<<nbk, BS_, lds, st>>>(A, H, Hlds);  \
+    else K_<M, AT, BS_, HUB_AGG_CAP / BS_, false><<<nbk, BS_, lds, st>>>(A, H, Hlds);     \
+  }
   if (H.segmented) {
     k_hub_count_seg<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
-    if (cnt)
-      k_hub_aggregate_seg<M, AT, HUB_AGG_IT, true><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
-    else
-      k_hub_aggregate_seg<M, AT, HUB_AGG_IT, false><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+    AGG_(k_hub_aggregate_seg, 256) AGG_(k_hub_aggregate_seg, 512) AGG_(k_hub_aggregate_seg, 1024)
   } else {
     launch_fill_bytes0(H.bcount, 8 * (size_t)H.nbuckets, st);
     k_hub_count<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
@@ -653,11 +666,9 @@ static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t
       k_hub_copy<<<(unsigned)gr, 256, 0, st>>>(H.boff, H.bcur, H.nbuckets);
     }
     k_hub_scatter<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
-    if (cnt)
-      k_hub_aggregate<M, AT, HUB_AGG_IT, true><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
-    else
-      k_hub_aggregate<M, AT, HUB_AGG_IT, false><<<(unsigned)H.nbuckets, 1024, lds, st>>>(A, H, Hlds);
+    AGG_(k_hub_aggregate, 256) AGG_(k_hub_aggregate, 512) AGG_(k_hub_aggregate, 1024)
   }
+#undef AGG_
   k_hub_final<M, AT><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
   return 0;
 }
@@ -678,17 +689,21 @@ int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStre
 }
 
 void hub_allow_lds(int bytes) {
-#define B_(K_, M_, AT_, C_)                                                  \
-  (void)hipFuncSetAttribute((const void*)K_<M_, AT_, HUB_AGG_IT, C_>,        \
+#define B_(K_, M_, AT_, BS_, C_)                                                       \
+  (void)hipFuncSetAttribute((const void*)K_<M_, AT_, BS_, HUB_AGG_CAP / BS_, C_>,      \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-#define A_(M_, AT_)                                                                   \
-  B_(k_hub_aggregate, M_, AT_, false) B_(k_hub_aggregate, M_, AT_, true)              \
-  B_(k_hub_aggregate_seg, M_, AT_, false) B_(k_hub_aggregate_seg, M_, AT_, true)
+#define S_(M_, AT_, BS_)                                                               \
+  B_(k_hub_aggregate, M_, AT_, BS_, false) B_(k_hub_aggregate, M_, AT_, BS_, true)     \
+  B_(k_hub_aggregate_seg, M_, AT_, BS_, false) B_(k_hub_aggregate_seg, M_, AT_, BS_, true)
+#define A_(M_, AT_) S_(M_, AT_, 256) S_(M_, AT_, 512) S_(M_, AT_, 1024)
   A_(ModeUnit, int32_t) A_(ModeUnit, int64_t) A_(ModeInt, int64_t) A_(ModeF64, double)
   A_(ModeF32, double) A_(ModeI32, int32_t) A_(ModeI32, int64_t)
 #undef A_
+#undef S_
+#undef B_
 }
 
 int hub_chunk_edges() { return HUB_CHUNK; }
 int hub_max_buckets() { return HUB_MAXP; }
 int hub_seg_max() { return HUB_SEG_MAX; }
+int hub_agg_cap() { return HUB_AGG_CAP; }

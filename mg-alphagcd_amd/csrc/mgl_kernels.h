@@ -116,12 +116,14 @@ struct HubPlan {
   const int64_t* hub_cbo0 = nullptr;    // [m] first cbo entry of each hub
   int32_t* cbo = nullptr;               // [sum over hubs of chunks x (P_i + 1)]
   unsigned long long* galloc = nullptr; // [1] HBM-table slots handed out in this pass
+  int agg_bs = 1024;                    // threads per aggregate workgroup: 256, 512 or 1024
 };
 int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStream_t st);
 void hub_allow_lds(int bytes);
 int hub_chunk_edges();
 int hub_max_buckets();
 int hub_seg_max();  // chunks per hub of the segmented form
+int hub_agg_cap();  // candidates of one bucket on the aggregate's owner path
 void launch_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out, hipStream_t st);
 
 // ---- per-vertex row reductions (vertex weights, modularity sum_in) ----------

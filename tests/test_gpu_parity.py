@@ -132,7 +132,7 @@ def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, bucket, segmented)
     # segmented = 0 forces the scatter form of the hub path
     g = hub_graph(mgl)
     tf = tmp_path / "tune.txt"
-    tf.write_text(f"hub_bucket {bucket}\nhub_segmented {segmented}\n" + "".join(
+    tf.write_text(f"hub_bucket_unclamped {bucket}\nhub_segmented {segmented}\n" + "".join(
         f"{u} {sh} {s_} {sz}\n" for u, sh, s_, sz in
         [(0, 0, 0, 0), (4, 1, 4, 3), (8, 1, 8, 4), (16, 1, 16, 5), (32, 1, 32, 6),
          (64, 1, 64, 7), (128, 1, 64, 8), (256, 1, 64, 9), (1024, 2, 256, 2048),
@@ -147,6 +147,29 @@ def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, bucket, segmented)
         c = rng.integers(0, g.n, g.n).astype(np.int32)
         if trial > 0:
             c = (c // (7 * trial)) * (7 * trial)
+        s.set_assign(c)
+        a = orc.community_totals(k, c, g.n)
+        for up in (True, False):
+            np.testing.assert_array_equal(
+                s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
+    s.close()
+
+
+@pytest.mark.parametrize("threads,per_cu", [(256, 2), (512, 4), (1024, 2), (1024, 1)])
+def test_hub_aggregate_workgroup_shapes(ctx, mgl, orc, tmp_path, threads, per_cu):
+    g = hub_graph(mgl)
+    tf = tmp_path / "tune.txt"
+    tf.write_text(f"hub_agg_threads {threads}\nhub_agg_per_cu {per_cu}\n" + "".join(
+        f"{u} {sh} {s_} {sz}\n" for u, sh, s_, sz in
+        [(0, 0, 0, 0), (4, 1, 4, 3), (8, 1, 8, 4), (16, 1, 16, 5), (32, 1, 32, 6),
+         (64, 1, 64, 7), (128, 1, 64, 8), (256, 1, 64, 9), (1024, 2, 256, 2048),
+         (-1, 3, 0, 0)]))
+    s = ctx.session(g, tuning_file=str(tf))
+    k = orc.vertex_weights(g.offsets)
+    rng = np.random.default_rng(9)
+    for trial in range(2):
+        c = rng.integers(0, g.n, g.n).astype(np.int32)
+        c = (c // (3 + 5 * trial)) * (3 + 5 * trial)
         s.set_assign(c)
         a = orc.community_totals(k, c, g.n)
         for up in (True, False):
