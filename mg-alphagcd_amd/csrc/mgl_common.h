@@ -147,7 +147,8 @@ __device__ __forceinline__ uint32_t probe_next(uint32_t h, uint32_t step, uint32
 // Total order used by the argmax (louvain.cu:544-554): higher gain first; on
 // equal gain the smaller id on up passes and the larger id on down passes
 // (tie_small forces "smaller" always).
-__device__ __forceinline__ bool better(double g, vid d, double bg, vid bd, bool prefer_small) {
+__host__ __device__ __forceinline__ bool better(double g, vid d, double bg, vid bd,
+                                                bool prefer_small) {
   return (g > bg) || (g == bg && (prefer_small ? d < bd : d > bd));
 }
 

@@ -410,9 +410,8 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
     const KT ki = ((const KT*)A.k)[v];
     const KT* a = (const KT*)A.a;
     const KT aci_k = a[ci] - ki;
-    const double Mh = A.gs[g].mass / 2.0;
-    const double deici = (double)((const KT*)H.hub_eici)[i];
-    const double dki = (double)ki, daci = (double)aci_k;
+    const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k,
+                                    ((const KT*)H.hub_eici)[i]);
     const AT* ba = (const AT*)src.a;
     if (cnt <= (int64_t)IT * BS && (uint64_t)cnt * 8 <= (uint64_t)Hmax * 7) {
       const uint32_t Ht = hub_table_size(cnt, Hmax);
@@ -439,7 +438,7 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
       for (int j = 0; j < IT; ++j)
         own[j] = key[j] != MGL_EMPTY ? tb.ins(key[j], wv[j], Ht, false, 0, pc) : -1;
       __syncthreads();
-      own_gains<M, AT, IT, TB>(tb, own, oa, deici, dki, daci, Mh, small, best, dst, occ);
+      own_gains<M, AT, IT, TB>(tb, own, oa, G, small, best, dst, occ);
     } else {
       bool done = false;
       if constexpr (PACK) {
@@ -470,6 +469,7 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
         }
         __syncthreads();
         if (!ovf) {
+          Argmax<M, AT> am(ci);
           for (uint32_t s = threadIdx.x; s < Hb; s += BS) {
             const uint64_t t = T[s];
             const vid cj = (vid)(uint32_t)(t >> 32);
@@ -478,9 +478,10 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
             ST ev;
             __builtin_memcpy(&ev, &lo, 4);
             ++occ;
-            const double wij = gain((double)ev, deici, dki, daci, (double)Avs[s], Mh);
-            if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+            am.offer(G, ev, Avs[s], cj, small);
           }
+          best = am.best;
+          dst = am.dst;
           done = true;
         }
         __syncthreads();  // the LDS table is not reused below, but ovf is read above
@@ -502,15 +503,17 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
         }
         __threadfence();
         __syncthreads();
+        Argmax<M, AT> am(ci);
         for (uint32_t s = threadIdx.x; s < Ht; s += BS) {
           const vid cj = __hip_atomic_load(&K[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           if (cj == MGL_EMPTY) continue;
           const ST ev = __hip_atomic_load(&V[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           const AT ak = __hip_atomic_load(&Av[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           ++occ;
-          const double wij = gain((double)ev, deici, dki, daci, (double)ak, Mh);
-          if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+          am.offer(G, ev, ak, cj, small);
         }
+        best = am.best;
+        dst = am.dst;
       }
     }
   }

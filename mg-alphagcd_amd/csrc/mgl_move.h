@@ -149,23 +149,22 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
   if (ok) {
     eici = group_sum<S>(eici);
     selfw = group_sum<S>(selfw);
-    const double Mh = A.gs[g].mass / 2.0;
     const KT ki = ((const KT*)A.k)[v];
     const KT aci_k = acik - ki;
-    const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
+    const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
     const bool small = A.tie_small ? true : (A.up != 0);
-    double best = 0.0;
-    vid dst = ci;
+    Argmax<M, AT> am(ci);
     int cnt = 0;
 #pragma unroll
     for (int i = lane; i < H; i += S) {
       const vid cj = K[i];
       if (cj != MGL_EMPTY) {
         ++cnt;
-        const double wij = gain((double)V[i], deici, dki, daci, (double)Av[i], Mh);
-        if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+        am.offer(G, V[i], Av[i], cj, small);
       }
     }
+    double best = am.best;
+    vid dst = am.dst;
     group_argmax<S>(best, dst, small);
     if (CNT && A.stats) {
       cnt = group_sum<S>(cnt);
@@ -221,22 +220,21 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
   selfw = block_sum<BS>(selfw, rs);
-  const double Mh = A.gs[g].mass / 2.0;
   const KT ki = ((const KT*)A.k)[v];
   const KT aci_k = (KT)rv.a - ki;
-  const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
+  const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
   const bool small = A.tie_small ? true : (A.up != 0);
-  double best = 0.0;
-  vid dst = ci;
+  Argmax<M, AT> am(ci);
   int cnt = 0;
   for (uint32_t i = threadIdx.x; i < H; i += BS) {
     const vid cj = K[i];
     if (cj != MGL_EMPTY) {
       ++cnt;
-      const double wij = gain((double)V[i], deici, dki, daci, (double)Av[i], Mh);
-      if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+      am.offer(G, V[i], Av[i], cj, small);
     }
   }
+  double best = am.best;
+  vid dst = am.dst;
   block_argmax<BS>(best, dst, small, rg, rd);
   if (CNT && A.stats) cnt = block_sum<BS>(cnt, rc);
   if (threadIdx.x == 0) {
@@ -327,15 +325,14 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
   if (ok) {
     eici = group_sum<S>(eici);
     selfw = group_sum<S>(selfw);
-    const double Mh = A.gs[g].mass / 2.0;
     const KT ki = ((const KT*)A.k)[v];
     const KT aci_k = acik - ki;
+    const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
     const bool small = A.tie_small ? true : (A.up != 0);
     double best = 0.0;
     vid dst = ci;
     int cnt = 0;
-    own_gains<M, AT, IT, TB>(tb, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small,
-                             best, dst, cnt);
+    own_gains<M, AT, IT, TB>(tb, own, oa, G, small, best, dst, cnt);
     group_argmax<S>(best, dst, small);
     if (CNT && A.stats) {
       cnt = group_sum<S>(cnt);
@@ -387,15 +384,14 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
   selfw = block_sum<BS>(selfw, rs);
-  const double Mh = A.gs[g].mass / 2.0;
   const KT ki = ((const KT*)A.k)[v];
   const KT aci_k = (KT)rv.a - ki;
+  const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
   const bool small = A.tie_small ? true : (A.up != 0);
   double best = 0.0;
   vid dst = ci;
   int cnt = 0;
-  own_gains<M, AT, IT, TB>(tb, own, oa, (double)eici, (double)ki, (double)aci_k, Mh, small, best,
-                           dst, cnt);
+  own_gains<M, AT, IT, TB>(tb, own, oa, G, small, best, dst, cnt);
   block_argmax<BS>(best, dst, small, rg, rd);
   if (CNT && A.stats) cnt = block_sum<BS>(cnt, rc);
   if (threadIdx.x == 0) {
@@ -465,21 +461,18 @@ __global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __rest
         }
       }
     }
-    const double Mh = A.gs[g].mass / 2.0;
     const KT ki = ((const KT*)A.k)[v];
     const KT aci_k = (KT)rv.a - ki;
-    const double deici = (double)eici, dki = (double)ki, daci = (double)aci_k;
+    const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
     const bool small = A.tie_small ? true : (A.up != 0);
-    double best = 0.0;
-    vid dst = ci;
+    Argmax<M, AT> am(ci);
 #pragma unroll
     for (int j = 0; j < D; ++j) {
       if (!cand[j]) continue;
       ++cnt;
-      const double wij = gain((double)w[j], deici, dki, daci, (double)r[j].a, Mh);
-      if (better(wij, r[j].c, best, dst, small)) { best = wij; dst = r[j].c; }
+      am.offer(G, w[j], r[j].a, r[j].c, small);
     }
-    A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
+    A.c_new[v] = clamp_dst(am.dst, ci, A.up, A.version);
     if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
   }
   if (CNT && A.stats) {

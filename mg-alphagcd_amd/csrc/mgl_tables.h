@@ -15,7 +15,7 @@ __device__ __forceinline__ void atom_add(double* p, double x) { atomicAdd(p, x);
 
 // Per-vertex gain of moving into cj (louvain.cu:540-543, same operation order):
 //   wij = e_{v,cj}; wij -= (eici - ki*(aci - acj)/(2*M)); wij /= M;  M = mass/2
-__device__ __forceinline__ double gain(double e, double eici, double ki, double aci,
+__host__ __device__ __forceinline__ double gain(double e, double eici, double ki, double aci,
                                        double acj, double Mh) {
   double wij = e;
   wij -= (eici - ki * (aci - acj) / (2. * Mh));
@@ -135,17 +135,104 @@ struct OwnTab<ST, false> {
   __device__ ST val(int i) const { return V[i]; }
 };
 
+// ---------------------------------------------------------------------------
+// Argmax over the candidate communities of one vertex (louvain.cu:526-556),
+// starting from (gain 0.0, own community).
+//
+// Exact prefilter, for the exact modes whose slot sums and community totals
+// are 32-bit (then 2m < 2^31): with D = 2m, m = D/2,
+//   X(c) = D*(e_c - eici) + ki*(aci - a_c)
+// is the exact value of gain(c) * D * m (an integer below 2^63). The FP64
+// expression of gain() rounds five times on terms bounded by ki (e_c, eici
+// <= ki and |aci - a_c| <= D), so it differs from X/(D*m) by less than
+// 8u*ki/m with u = 2^-53, and two exact values more than 2u apart, relative,
+// round to distinct doubles. With delta = D*ki/2^48 (= 32u*D*ki) therefore
+//   X(c) < X(b) - delta  =>  gain(c) < gain(b)  strictly,
+// where b is the current best (X = 0 and gain 0.0 for staying). Such a
+// candidate cannot be the argmax and its gain() is not evaluated. A candidate
+// with the same (e, a) as the current best has the same gain; only the ids
+// are compared. Every other candidate is decided by gain() as before, so the
+// result is bit-for-bit that of evaluating every candidate; the prefilter
+// only saves the FP64 work (two correctly rounded divisions per candidate).
+struct GainIn {
+  double deici, dki, daci, Mh;  // FP64 inputs of gain()
+  int64_t D, ki, aci, eici;     // exact inputs of the prefilter
+  int64_t delta;
+};
+template <class M, class AT>
+__host__ __device__ constexpr bool gain_prefilter() {
+  return M::EXACT && sizeof(typename M::S) == 4 && sizeof(AT) == 4;
+}
+
+// Inputs of a vertex's gains: 2m as double and (exact modes) as an integer,
+// the vertex's total ki, its own community's total without it (aci) and its
+// weight to its own community without self-loops (eici).
+template <class M, class AT, class KT, class ST>
+__host__ __device__ __forceinline__ GainIn gain_in(double mass, int64_t mass_i, KT ki, KT aci_k, ST eici) {
+  GainIn g;
+  g.Mh = mass / 2.0;
+  g.deici = (double)eici;
+  g.dki = (double)ki;
+  g.daci = (double)aci_k;
+  g.D = g.ki = g.aci = g.eici = g.delta = 0;
+  if constexpr (gain_prefilter<M, AT>()) {
+    g.D = mass_i;
+    g.ki = (int64_t)ki;
+    g.aci = (int64_t)aci_k;
+    g.eici = (int64_t)eici;
+    g.delta = ((mass_i * (int64_t)ki) >> 48) + 1;  // mass_i, ki < 2^31
+  }
+  return g;
+}
+
+template <class M, class AT>
+struct Argmax {
+  typedef typename M::S ST;
+  static constexpr bool PRE = gain_prefilter<M, AT>();
+  double best = 0.0;
+  vid dst;
+  int64_t xb = 0;  // X of the current best
+  ST eb = 0;
+  AT ab = 0;
+  bool have = false;  // the current best is a candidate (not the own community)
+  __host__ __device__ explicit Argmax(vid ci) : dst(ci) {}
+  __host__ __device__ __forceinline__ void offer(const GainIn& G, ST e, AT a, vid cj,
+                                                      bool small) {
+    if constexpr (PRE) {
+      const int64_t x = (int64_t)(int32_t)G.D * (int64_t)((int32_t)e - (int32_t)G.eici) +
+                        (int64_t)(int32_t)G.ki * (int64_t)((int32_t)G.aci - (int32_t)a);
+      if (x < xb - G.delta) return;
+      if (have && e == eb && a == ab) {
+        if (small ? cj < dst : cj > dst) dst = cj;
+        return;
+      }
+      const double g = gain((double)e, G.deici, G.dki, G.daci, (double)a, G.Mh);
+      if (better(g, cj, best, dst, small)) {
+        best = g;
+        dst = cj;
+        xb = x;
+        eb = e;
+        ab = a;
+        have = true;
+      }
+    } else {
+      const double g = gain((double)e, G.deici, G.dki, G.daci, (double)a, G.Mh);
+      if (better(g, cj, best, dst, small)) { best = g; dst = cj; }
+    }
+  }
+};
+
 template <class M, class AT, int IT, class TB>
 __device__ __forceinline__ void own_gains(const TB& tb, const int (&own)[IT], const AT (&oa)[IT],
-                                          double deici, double dki, double daci, double Mh,
-                                          bool small, double& best, vid& dst, int& cnt) {
+                                          const GainIn& G, bool small, double& best, vid& dst,
+                                          int& cnt) {
+  Argmax<M, AT> am(dst);
 #pragma unroll
   for (int j = 0; j < IT; ++j) {
     if (own[j] < 0) continue;
     ++cnt;
-    const vid cj = tb.key(own[j]);
-    const double wij = gain((double)tb.val(own[j]), deici, dki, daci, (double)oa[j], Mh);
-    if (better(wij, cj, best, dst, small)) { best = wij; dst = cj; }
+    am.offer(G, tb.val(own[j]), oa[j], tb.key(own[j]), small);
   }
+  best = am.best;
+  dst = am.dst;
 }
-

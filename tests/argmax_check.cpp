@@ -1,0 +1,150 @@
+// Host check of the exact gain prefilter of the move kernels (Argmax in
+// mg-alphagcd_amd/csrc/mgl_tables.h, compiled here for the CPU): over random
+// candidate sets, many of them built around near-ties of the exact gain
+// numerator and exact duplicates, the prefiltered argmax must pick the same
+// community with the same double as evaluating gain() for every candidate
+// (the reference's order, louvain.cu:526-556). Prints the number of
+// mismatches and exits non-zero if there is one.
+//
+// Usage: argmax_check SEED CASES
+#include "mgl_tables.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <vector>
+
+struct Cand {
+  int32_t e, a;
+  vid id;
+};
+
+// x^-1 mod m (gcd(x, m) = 1), by the extended Euclidean algorithm.
+static int64_t inv_mod(int64_t x, int64_t m) {
+  __int128 a = x % m, b = m, u = 1, v = 0;
+  while (b) {
+    const __int128 q = a / b, t = a - q * b;
+    a = b;
+    b = t;
+    const __int128 w = u - q * v;
+    u = v;
+    v = w;
+  }
+  if (a != 1) return -1;
+  __int128 r = u % m;
+  if (r < 0) r += m;
+  return (int64_t)r;
+}
+static int64_t gcd64(int64_t a, int64_t b) {
+  while (b) {
+    const int64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+int main(int argc, char** argv) {
+  const uint64_t seed = argc > 1 ? strtoull(argv[1], nullptr, 10) : 1;
+  const long cases = argc > 2 ? atol(argv[2]) : 100000;
+  std::mt19937_64 rng(seed);
+  auto uni = [&](int64_t lo, int64_t hi) {  // inclusive
+    return lo + (int64_t)(rng() % (uint64_t)(hi - lo + 1));
+  };
+  auto logu = [&](int64_t lo, int64_t hi) {  // log-uniform in [lo, hi]
+    const double x = std::exp(std::log((double)lo) +
+                              (std::log((double)hi) - std::log((double)lo)) *
+                                  ((double)(rng() >> 11) * 0x1.0p-53));
+    int64_t v = (int64_t)x;
+    return v < lo ? lo : (v > hi ? hi : v);
+  };
+  long bad = 0, skipped = 0, total = 0;
+  std::vector<Cand> cs;
+  for (long t = 0; t < cases; ++t) {
+    // a third of the cases: large 2m and ki, candidates whose exact gain
+    // numerators X differ from the first one's by a few units (constructed
+    // below), where rounding can reorder or tie the FP64 gains
+    const int mode = (int)uni(0, 2);
+    const int64_t D = mode == 2 ? uni((int64_t)1 << 28, ((int64_t)1 << 31) - 1)
+                                : logu(2, ((int64_t)1 << 31) - 1);  // 2m
+    const int64_t ki = mode == 2 ? uni(D / 4, D / 2) : logu(1, D);
+    const int64_t eici = uni(0, ki / 2);
+    const int64_t aci = uni(0, D - ki);
+    const int n = (int)uni(1, 40);
+    const bool near = mode == 1;
+    const bool exact_near = mode == 2 && gcd64(D, ki) == 1;
+    const bool small = rng() & 1;
+    cs.clear();
+    const int64_t emax = ki - eici > 0 ? ki - eici : 1;
+    int32_t e0 = (int32_t)logu(1, emax);
+    int32_t a0 = (int32_t)uni(0, D - ki);
+    if (exact_near) {
+      e0 = (int32_t)uni(emax / 4 + 1, emax / 2 + 1);
+      a0 = (int32_t)uni((D - ki) / 4, (D - ki) / 2);
+    }
+    const int64_t Dinv = exact_near ? inv_mod(D % ki, ki) : -1;
+    for (int c = 0; c < n; ++c) {
+      Cand x;
+      x.id = (vid)(c * 7919 + (int)uni(0, 6000));
+      if (exact_near && c > 0) {
+        // X(e0 + de, a0 + da) - X(e0, a0) = D*de - ki*da = -k: de = -k D^-1 mod ki
+        x.e = e0;
+        x.a = a0;
+        for (int tries = 0; tries < 64; ++tries) {
+          const int64_t k = uni(-40, 40);
+          __int128 de = (__int128)(((-k % ki) + ki) % ki) * Dinv % ki;  // in [0, ki)
+          if (uni(0, 1)) de -= ki;                                         // or in [-ki, 0)
+          const __int128 num = (__int128)D * de + k;
+          if (num % ki != 0) continue;
+          const __int128 e = e0 + de, a = a0 + num / ki;
+          if (e < 1 || e > emax || a < 0 || a > D - ki) continue;
+          x.e = (int32_t)e;
+          x.a = (int32_t)a;
+          break;
+        }
+      } else if (near && c > 0) {
+        // (e0 + de, a0 + da) with D*de ~ ki*da: X within a few ki of X(e0, a0)
+        const int64_t de = uni(-3, 3);
+        int64_t e = e0 + de;
+        if (e < 1) e = 1;
+        if (e > emax) e = emax;
+        int64_t da = (int64_t)std::llround((double)(e - e0) * (double)D / (double)ki) + uni(-3, 3);
+        int64_t a = a0 + da;
+        if (a < 0) a = 0;
+        if (a > D - ki) a = D - ki;
+        if (uni(0, 7) == 0) { e = e0; a = a0; }  // exact duplicate of the base
+        x.e = (int32_t)e;
+        x.a = (int32_t)a;
+      } else {
+        x.e = (int32_t)logu(1, emax);
+        x.a = (int32_t)uni(0, D - ki);
+      }
+      cs.push_back(x);
+    }
+    const double mass = (double)D;
+    const GainIn G = gain_in<ModeUnit, int32_t>(mass, D, (int64_t)ki, (int64_t)aci, (int32_t)eici);
+    // every candidate evaluated, in the reference's order
+    double best = 0.0;
+    vid dst = (vid)uni(0, 1 << 30);
+    const vid ci = dst;
+    for (const Cand& x : cs) {
+      const double g = gain((double)x.e, G.deici, G.dki, G.daci, (double)x.a, G.Mh);
+      if (better(g, x.id, best, dst, small)) { best = g; dst = x.id; }
+    }
+    Argmax<ModeUnit, int32_t> am(ci);
+    for (const Cand& x : cs) am.offer(G, x.e, x.a, x.id, small);
+    total += n;
+    if (am.dst != dst || std::memcmp(&am.best, &best, sizeof(double)) != 0) {
+      if (bad < 5)
+        printf("mismatch: D=%lld ki=%lld eici=%lld aci=%lld n=%d near=%d: full (%d, %.17g) "
+               "prefiltered (%d, %.17g)\n",
+               (long long)D, (long long)ki, (long long)eici, (long long)aci, n, (int)near,
+               (int)dst, best, (int)am.dst, am.best);
+      ++bad;
+    }
+    (void)skipped;
+  }
+  printf("cases %ld candidates %ld mismatches %ld\n", cases, total, bad);
+  return bad ? 1 : 0;
+}

@@ -125,3 +125,23 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_gain_prefilter_matches_full_evaluation(tmp_path):
+    # the move kernels' exact gain prefilter (Argmax in mgl_tables.h), built for
+    # the CPU: on random candidate sets and constructed near-ties of the exact
+    # gain numerator, the same argmax and the same double as evaluating every
+    # candidate's FP64 gain
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    exe = tmp_path / "argmax_check"
+    subprocess.run([hipcc, "-O2", "-std=c++17", "-ffp-contract=off",
+                    "-I", os.path.join(ROOT, "mg-alphagcd_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "argmax_check.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True, timeout=300)
+    for seed in (1, 2, 3):
+        r = subprocess.run([str(exe), str(seed), "300000"], capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout
+        assert "mismatches 0" in r.stdout
