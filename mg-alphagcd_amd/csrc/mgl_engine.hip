@@ -528,8 +528,11 @@ struct Engine {
     LevelGraph& g = LG();
     MoveArgs A = move_args(from, to, up, vin);
     const int r32 = rec32_now() ? 1 : 0;
+    // in degree-bin order the isolated vertices (bin 0) take the first
+    // positions; no CSR entry points to them and they are never moved
+    const int64_t first = (rlay && nlist < 0 && L.nb > 1) ? bin_off[1] : 0;
     launch_build_rec(g.mode, r32, vcount(), from, ba.p, brec.p, st(), vlist(),
-                     rlay ? brpos.as<vid>() : nullptr, rlay ? brinv.as<vid>() : nullptr);
+                     rlay ? brpos.as<vid>() : nullptr, rlay ? brinv.as<vid>() : nullptr, first);
     int ne = 0;
     for (int b = L.nb - 1; b >= 1; --b) {
       if (bin_n[b] == 0) continue;

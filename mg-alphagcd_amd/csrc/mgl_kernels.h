@@ -73,9 +73,11 @@ MGL_MOVE_UNIT_DECL(f32)
 // list (optional): the n vertex ids to process instead of 0..n-1
 // rpos / rinv (optional): record position of each vertex and its inverse; with
 // rinv and no list, position i is written from vertex rinv[i] (sequential writes)
+// first: items before it are skipped (positions of isolated vertices, whose
+// records no move kernel reads)
 void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
                       hipStream_t st, const vid* list = nullptr, const vid* rpos = nullptr,
-                      const vid* rinv = nullptr);
+                      const vid* rinv = nullptr, int64_t first = 0);
 // Record layout of a level: rpos[perm[i]] = i and rinv[i] = perm[i] for i < n;
 // colr[e] = rpos[col[e]] for e < nnz.
 void launch_record_layout(const vid* perm, int64_t n, vid* rpos, vid* rinv, hipStream_t st);

@@ -38,13 +38,14 @@ static unsigned grid_for(int64_t n, int per_block, unsigned cap) {
 // Records of a large level are stored in degree-bin order (rpos): the
 // frequently gathered records of high-degree vertices then share cache lines.
 template <class KT, class AT>
-__global__ __launch_bounds__(256) void k_build_rec(int64_t n, const vid* __restrict__ c,
+__global__ __launch_bounds__(256) void k_build_rec(int64_t first, int64_t n,
+                                                   const vid* __restrict__ c,
                                                    const KT* __restrict__ a,
                                                    Rec<AT>* __restrict__ rec,
                                                    const vid* __restrict__ list,
                                                    const vid* __restrict__ rpos,
                                                    const vid* __restrict__ rinv) {
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+  for (int64_t i = first + (int64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * 256) {
     int64_t v, at;
     if (list) {
@@ -66,18 +67,19 @@ __global__ __launch_bounds__(256) void k_build_rec(int64_t n, const vid* __restr
 }
 
 void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
-                      hipStream_t st, const vid* list, const vid* rpos, const vid* rinv) {
-  if (n <= 0) return;
-  const unsigned grid = grid_for(n, 256, 8192);
+                      hipStream_t st, const vid* list, const vid* rpos, const vid* rinv,
+                      int64_t first) {
+  if (n - first <= 0) return;
+  const unsigned grid = grid_for(n - first, 256, 8192);
   if ((mode == MODE_UNIT || mode == MODE_I32) && rec32)
-    k_build_rec<int64_t, int32_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a,
+    k_build_rec<int64_t, int32_t><<<grid, 256, 0, st>>>(first, n, c, (const int64_t*)a,
                                                         (Rec<int32_t>*)rec, list, rpos, rinv);
   else if (mode_is_int(mode))
-    k_build_rec<int64_t, int64_t><<<grid, 256, 0, st>>>(n, c, (const int64_t*)a,
+    k_build_rec<int64_t, int64_t><<<grid, 256, 0, st>>>(first, n, c, (const int64_t*)a,
                                                         (Rec<int64_t>*)rec, list, rpos, rinv);
   else
-    k_build_rec<double, double><<<grid, 256, 0, st>>>(n, c, (const double*)a, (Rec<double>*)rec,
-                                                      list, rpos, rinv);
+    k_build_rec<double, double><<<grid, 256, 0, st>>>(first, n, c, (const double*)a,
+                                                      (Rec<double>*)rec, list, rpos, rinv);
 }
 
 __global__ __launch_bounds__(256) void k_record_layout(const vid* __restrict__ perm, int64_t n,
