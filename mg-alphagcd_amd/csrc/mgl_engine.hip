@@ -519,6 +519,7 @@ struct Engine {
     A.tie_small = o.tie_rule == MGL_TIE_SMALLEST;
     A.stats = nullptr;
     A.probes = nullptr;
+    A.pos0 = -1;
     return A;
   }
 
@@ -541,6 +542,7 @@ struct Engine {
         A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS + b;
         A.probes = bcounts.as<unsigned long long>() + 4 * MGL_MAX_BINS;
       }
+      A.pos0 = (rlay && nlist < 0 && L.b[b].shape != SHAPE_GLOBAL) ? bin_off[b] : -1;
       if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b], st()));
       const int rc = (L.b[b].shape == SHAPE_GLOBAL)
                          ? launch_hub(g.mode, r32, A, hub, st())

@@ -48,6 +48,8 @@ struct MoveArgs {
   int tie_small;
   unsigned long long* stats; // [0] distinct pairs (nullable)
   unsigned long long* probes; // [3] LDS hash inserts, probes, multi-probe inserts (nullable)
+  int64_t pos0;       // >= 0: perm[i] has record position pos0 + i (records in degree-bin
+                      // order, bins not compacted); < 0: look the position up in rpos
 };
 
 // Launch the move kernel of one GROUP or BLOCK bin. perm points at the bin's

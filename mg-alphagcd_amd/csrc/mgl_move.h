@@ -29,9 +29,11 @@ __device__ __forceinline__ vid clamp_dst(vid dst, vid ci, int up, int version) {
 }
 
 
-// Record position of v (its own record and the id its self-loop carries in A.col).
-__device__ __forceinline__ vid rec_pos(const MoveArgs& A, vid v) {
-  return A.rpos ? A.rpos[v] : v;
+// Record position of v = perm[gi] (its own record and the id its self-loop
+// carries in A.col): in degree-bin order the bin's vertices hold consecutive
+// positions, so neither the rpos gather nor a scattered record read is needed.
+__device__ __forceinline__ vid rec_pos(const MoveArgs& A, vid v, int64_t gi) {
+  return A.pos0 >= 0 ? (vid)(A.pos0 + gi) : (A.rpos ? A.rpos[v] : v);
 }
 
 // Insert (key, w) into a table of 2^lg slots; the creator of a key also
@@ -138,7 +140,7 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
   KT acik = 0;
   ProbeCount pc;
   if (ok) {
-    const vid vp = rec_pos(A, v);
+    const vid vp = rec_pos(A, v, gi);
     const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
     ci = rv.c;
     acik = (KT)rv.a;
@@ -208,7 +210,7 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   ST* V = (ST*)(smem + ((4 * (size_t)H + 15) / 16) * 16);
   AT* Av = (AT*)((unsigned char*)V + ((sizeof(ST) * (size_t)H + 15) / 16) * 16);
   for (uint32_t i = threadIdx.x; i < H; i += BS) { K[i] = MGL_EMPTY; V[i] = (ST)0; }
-  const vid vp = rec_pos(A, v);
+  const vid vp = rec_pos(A, v, gi);
   const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
   const vid ci = rv.c;
   __syncthreads();
@@ -314,7 +316,7 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
 #pragma unroll
   for (int j = 0; j < IT; ++j) own[j] = -1;
   if (ok) {
-    const vid vp = rec_pos(A, v);
+    const vid vp = rec_pos(A, v, gi);
     const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
     ci = rv.c;
     acik = (KT)rv.a;
@@ -370,7 +372,7 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   TB tb;
   tb.init(smem, H);
   for (uint32_t i = threadIdx.x; i < H; i += BS) tb.clear(i);
-  const vid vp = rec_pos(A, v);
+  const vid vp = rec_pos(A, v, gi);
   const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
   const vid ci = rv.c;
   __syncthreads();
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __rest
   int cnt = 0;
   if (ok) {
     const Rec<AT>* R = (const Rec<AT>*)A.rec;
-    const vid vp = rec_pos(A, v);
+    const vid vp = rec_pos(A, v, gi);
     const Rec<AT> rv = R[vp];
     const vid ci = rv.c;
     const eid e0 = A.off[v];
