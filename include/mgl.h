@@ -218,6 +218,20 @@ mgl_status mgl_gen_sbm(int64_t n, int64_t n_comm, double exponent, int64_t min_s
 mgl_status mgl_read_mtx(const char* path, mgl_host_csr* out);
 /* Writes assignment / CSR files (text). */
 mgl_status mgl_write_assign(const char* path, const int32_t* assign, int64_t n);
+/* Writes a result to files named from prefix (no reference counterpart: the
+ * reference prints the per-level table only, louvain.cu:1536-1538,1654,1670-1703):
+ *   prefix.levels.txt      level, vertices, iterations, Q, Q trajectory
+ *   prefix.final.{txt,bin} composed assignment of the input vertices
+ *   prefix.level<L>.{txt,bin}  the dendrogram: assignment of level L's vertices
+ *                          (community ids of level L; level L+1's vertex of a
+ *                          community is its rank among the used ids)   [keep_levels]
+ *   prefix.level<L>.{mtx,csr} coarse graph that is the input of level L >= 1:
+ *                          Matrix Market "coordinate real general" with every
+ *                          entry, or binary CSR                          [keep_graphs]
+ * binary = 0: text; 1: binary, little-endian: an 8-byte tag ("MGLASG1" or
+ * "MGLCSR1" and a zero byte), int64 n (and int64 nnz), then int32 labels, or
+ * int64 offsets[n+1], int32 cols[nnz], float64 weights[nnz]. */
+mgl_status mgl_write_result(const mgl_result* r, const char* prefix, int binary);
 void mgl_csr_free(mgl_host_csr* g);
 
 /* Build-time / tuning helpers. */

@@ -132,6 +132,7 @@ _sig("mgl_gen_rmat", _i, [_i, _d, _d, _d, _d, C.c_uint64, _i, _i64, C.POINTER(_H
 _sig("mgl_gen_sbm", _i, [_i64, _i64, _d, _i64, _d, _d, C.c_uint64, _i, C.POINTER(_HostCsr)])
 _sig("mgl_read_mtx", _i, [C.c_char_p, C.POINTER(_HostCsr)])
 _sig("mgl_write_assign", _i, [C.c_char_p, C.POINTER(_i32), _i64])
+_sig("mgl_write_result", _i, [C.c_void_p, C.c_char_p, _i])
 _sig("mgl_csr_free", None, [C.POINTER(_HostCsr)])
 _sig("mgl_version_major", _i, [])
 _sig("mgl_bin_bounds", _i, [_vp, C.POINTER(_i64), _i])
@@ -252,7 +253,12 @@ class Result:
     seconds: float = 0.0
 
 
-def _read_result(r) -> Result:
+def _read_result(r, dump: Optional[str] = None, dump_binary: bool = False) -> Result:
+    if dump:
+        st = _lib.mgl_write_result(r, dump.encode(), 1 if dump_binary else 0)
+        if st != OK:
+            _lib.mgl_result_free(r)
+            raise MglError(st, f"mgl_write_result({dump!r}) failed")
     out = Result()
     L = _lib.mgl_result_levels(r)
     for l in range(L):
@@ -329,12 +335,15 @@ class Context:
         if st != OK:
             raise MglError(st, _lib.mgl_last_error(self._h).decode(errors="replace"))
 
-    def louvain(self, g: CSR, **kw) -> Result:
+    def louvain(self, g: CSR, dump: Optional[str] = None, dump_binary: bool = False,
+                **kw) -> Result:
+        """Full multi-level run; with dump=PREFIX the result files are written
+        (mgl_write_result: levels, final assignment, dendrogram, coarse graphs)."""
         o = make_opts(**kw)
         cg = g._c()
         r = C.c_void_p()
         self._err(_lib.mgl_louvain(self._h, C.byref(cg), C.byref(o), C.byref(r)))
-        return _read_result(r)
+        return _read_result(r, dump, dump_binary)
 
     def louvain_batched(self, graphs: Sequence[CSR], **kw) -> List[Result]:
         o = make_opts(**kw)

@@ -6,6 +6,7 @@
 //   mgl_cli -path_of_graph G.mtx [-threshold 1e-6] [-max_phases 1000]
 //           [-max_iter 500] [-version 2] [-precision auto|exact|fp64|fp32]
 //           [-honour_max_iter 0|1] [-out labels.txt] [-device 0]
+//           [-dump PREFIX [-dump_binary 0|1]]   levels, dendrogram, coarse graphs
 //   mgl_cli -rmat SCALE,EF,SEED [...]      synthetic R-MAT (0.57,0.19,0.19,0.05)
 //   mgl_cli -sbm N,COMMUNITIES,SEED [...]  synthetic planted partition
 //
@@ -34,6 +35,8 @@ int main(int argc, char** argv) {
   const int honour = atoi(arg(argc, argv, "-honour_max_iter", "0"));
   const std::string prec = arg(argc, argv, "-precision", "auto");
   const char* out = arg(argc, argv, "-out", nullptr);
+  const char* dump = arg(argc, argv, "-dump", nullptr);
+  const int dump_binary = atoi(arg(argc, argv, "-dump_binary", "0"));
   const int device = atoi(arg(argc, argv, "-device", "0"));
   const char* rmat = arg(argc, argv, "-rmat", nullptr);
   const char* sbm = arg(argc, argv, "-sbm", nullptr);
@@ -87,6 +90,7 @@ int main(int argc, char** argv) {
               : prec == "fp64"  ? MGL_PREC_FP64
               : prec == "fp32"  ? MGL_PREC_FP32_HASH
                                 : MGL_PREC_AUTO;
+  if (dump) o.keep_levels = o.keep_graphs = 1;
   mgl_csr g = {h.n, h.nnz, h.offsets, h.cols, h.w};
   mgl_result* r = nullptr;
   st = mgl_louvain(ctx, &g, &o, &r);
@@ -129,6 +133,8 @@ int main(int argc, char** argv) {
     if (mgl_write_assign(out, mgl_result_final_assign(r), h.n) != MGL_OK)
       fprintf(stderr, "cannot write %s\n", out);
   }
+  if (dump && mgl_write_result(r, dump, dump_binary ? 1 : 0) != MGL_OK)
+    fprintf(stderr, "cannot write the result files %s.*\n", dump);
   mgl_result_free(r);
   mgl_destroy(ctx);
   mgl_csr_free(&h);

@@ -429,4 +429,79 @@ mgl_status mgl_write_assign(const char* path, const int32_t* assign, int64_t n) 
   return MGL_OK;
 }
 
+static mgl_status write_assign_bin(const char* path, const int32_t* a, int64_t n) {
+  FILE* f = fopen(path, "wb");
+  if (!f) return MGL_INVALID_ARG;
+  const char tag[8] = {'M', 'G', 'L', 'A', 'S', 'G', '1', 0};
+  bool ok = fwrite(tag, 1, 8, f) == 8 && fwrite(&n, 8, 1, f) == 1 &&
+            (n == 0 || fwrite(a, 4, (size_t)n, f) == (size_t)n);
+  ok = (fclose(f) == 0) && ok;
+  return ok ? MGL_OK : MGL_INVALID_ARG;
+}
+
+static mgl_status write_graph(const char* path, int binary, int64_t n, int64_t nnz,
+                              const int64_t* off, const int32_t* col, const double* w) {
+  FILE* f = fopen(path, binary ? "wb" : "w");
+  if (!f) return MGL_INVALID_ARG;
+  bool ok = true;
+  if (binary) {
+    const char tag[8] = {'M', 'G', 'L', 'C', 'S', 'R', '1', 0};
+    ok = fwrite(tag, 1, 8, f) == 8 && fwrite(&n, 8, 1, f) == 1 && fwrite(&nnz, 8, 1, f) == 1 &&
+         fwrite(off, 8, (size_t)n + 1, f) == (size_t)n + 1 &&
+         (nnz == 0 || (fwrite(col, 4, (size_t)nnz, f) == (size_t)nnz &&
+                       fwrite(w, 8, (size_t)nnz, f) == (size_t)nnz));
+  } else {
+    ok = fprintf(f, "%%%%MatrixMarket matrix coordinate real general\n%lld %lld %lld\n",
+                 (long long)n, (long long)n, (long long)nnz) > 0;
+    for (int64_t v = 0; ok && v < n; ++v)
+      for (int64_t e = off[v]; ok && e < off[v + 1]; ++e)
+        ok = fprintf(f, "%lld %d %.17g\n", (long long)(v + 1), col[e] + 1, w[e]) > 0;
+  }
+  ok = (fclose(f) == 0) && ok;
+  return ok ? MGL_OK : MGL_INVALID_ARG;
+}
+
+mgl_status mgl_write_result(const mgl_result* r, const char* prefix, int binary) {
+  if (!r || !prefix || binary < 0 || binary > 1) return MGL_INVALID_ARG;
+  const std::string p = prefix;
+  const char* ax = binary ? ".bin" : ".txt";
+  const int L = mgl_result_levels(r);
+  FILE* f = fopen((p + ".levels.txt").c_str(), "w");
+  if (!f) return MGL_INVALID_ARG;
+  fprintf(f, "# level vertices iterations Q trajectory...\n");
+  for (int l = 0; l < L; ++l) {
+    const int it = mgl_result_level_iters(r, l);
+    fprintf(f, "%d %lld %d %.17g", l, (long long)mgl_result_level_n(r, l), it,
+            mgl_result_level_q(r, l));
+    const double* qt = mgl_result_level_qtraj(r, l);
+    for (int i = 0; qt && i < it; ++i) fprintf(f, " %.17g", qt[i]);
+    fprintf(f, "\n");
+  }
+  if (fclose(f) != 0) return MGL_INVALID_ARG;
+  const int64_t n0 = L > 0 ? mgl_result_level_n(r, 0) : 0;
+  const int32_t* fin = mgl_result_final_assign(r);
+  mgl_status s = binary ? write_assign_bin((p + ".final" + ax).c_str(), fin, fin ? n0 : 0)
+                        : mgl_write_assign((p + ".final" + ax).c_str(), fin, fin ? n0 : 0);
+  if (s) return s;
+  for (int l = 0; l < L; ++l) {
+    const std::string base = p + ".level" + std::to_string(l);
+    const int32_t* a = mgl_result_level_assign(r, l);
+    if (a) {
+      const int64_t n = mgl_result_level_n(r, l);
+      s = binary ? write_assign_bin((base + ax).c_str(), a, n)
+                 : mgl_write_assign((base + ax).c_str(), a, n);
+      if (s) return s;
+    }
+    int64_t n = 0, nnz = 0;
+    const int64_t* off = nullptr;
+    const int32_t* col = nullptr;
+    const double* w = nullptr;
+    if (l >= 1 && mgl_result_level_graph(r, l, &n, &nnz, &off, &col, &w) == MGL_OK) {
+      s = write_graph((base + (binary ? ".csr" : ".mtx")).c_str(), binary, n, nnz, off, col, w);
+      if (s) return s;
+    }
+  }
+  return MGL_OK;
+}
+
 }  // extern "C"
