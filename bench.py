@@ -305,7 +305,10 @@ def run_rmat(args, mgl):
                           tuning_file=args.tuning_file, record_layout=args.record_layout)
         full = dict(seconds=res.seconds, host_seconds_incl_upload=time.perf_counter() - t0,
                     levels=len(res.levels), q=res.final_q,
-                    level_iters=[l.iters for l in res.levels])
+                    level_iters=[l.iters for l in res.levels],
+                    level_vertices=[l.n for l in res.levels],
+                    level_seconds=[round(l.seconds, 6) for l in res.levels],
+                    level_end_seconds=[round(l.coarsen_seconds, 6) for l in res.levels])
         log(f"full run: {full}")
     ctx.close()
     cpu = None

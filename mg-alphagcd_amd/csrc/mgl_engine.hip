@@ -2180,6 +2180,29 @@ static std::vector<BinLayout> tuning_candidates(const mgl_ctx* ctx) {
   return out;
 }
 
+// Median move time (ms) of passes 2-4 from singletons under the context's
+// current tuning.
+static mgl_status tune_time(mgl_ctx* ctx, const mgl_csr* g, double& med) {
+  mgl_opts o;
+  mgl_opts_default(&o);
+  mgl_session* s = nullptr;
+  mgl_status st = mgl_session_create(ctx, g, &o, &s);
+  if (st) return st;
+  double q;
+  int acc;
+  std::vector<double> t;
+  st = mgl_session_reset(s, &q);
+  for (int i = 0; i < 4 && st == MGL_OK; ++i) {
+    st = mgl_session_pass(s, &q, &acc);
+    if (st == MGL_OK && i > 0) t.push_back(s->st.move_ms);
+  }
+  mgl_session_destroy(s);
+  if (st) return st;
+  std::sort(t.begin(), t.end());
+  med = t.empty() ? 1e300 : t[t.size() / 2];
+  return MGL_OK;
+}
+
 extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* path) {
   if (!ctx) return MGL_INVALID_ARG;
   if (!g || !path) FAIL(ctx, MGL_INVALID_ARG, "null argument");
@@ -2189,25 +2212,31 @@ extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* p
   for (const BinLayout& cand : tuning_candidates(ctx)) {
     ctx->tune.bins = cand;
     ctx->tune.have_bins = true;
-    mgl_opts o;
-    mgl_opts_default(&o);
-    mgl_session* s = nullptr;
-    mgl_status st = mgl_session_create(ctx, g, &o, &s);
+    double med = 0;
+    const mgl_status st = tune_time(ctx, g, med);
     if (st) { ctx->tune = saved; return st; }
-    double q;
-    int acc;
-    std::vector<double> t;
-    st = mgl_session_reset(s, &q);
-    for (int i = 0; i < 4 && st == MGL_OK; ++i) {
-      st = mgl_session_pass(s, &q, &acc);
-      if (st == MGL_OK && i > 0) t.push_back(s->st.move_ms);
-    }
-    mgl_session_destroy(s);
-    if (st) { ctx->tune = saved; return st; }
-    std::sort(t.begin(), t.end());
-    const double med = t.empty() ? 1e300 : t[t.size() / 2];
     if (med < best_ms) { best_ms = med; best = cand; }
   }
+  // then the hub path's aggregate shape on the best layout (threads per
+  // workgroup, workgroups per CU, candidates per bucket); without hub rows
+  // every candidate times the same and the first one stays
+  struct HubShape { int threads, per_cu; int64_t bucket; };
+  const HubShape hubs[] = {{512, 4, 3072}, {1024, 2, 4096}, {512, 4, 4096}, {512, 2, 4096},
+                           {1024, 2, 3072}, {256, 4, 2048}};
+  ctx->tune.bins = best;
+  ctx->tune.have_bins = true;
+  HubShape hbest = hubs[0];
+  double hbest_ms = 1e300;
+  for (const HubShape& h : hubs) {
+    ctx->tune.hub_agg_threads = h.threads;
+    ctx->tune.hub_agg_per_cu = h.per_cu;
+    ctx->tune.hub_bucket = h.bucket;
+    double med = 0;
+    const mgl_status st = tune_time(ctx, g, med);
+    if (st) { ctx->tune = saved; return st; }
+    if (med < hbest_ms * 0.99) { hbest_ms = med; hbest = h; }
+  }
+  best_ms = std::min(best_ms, hbest_ms);
   ctx->tune = saved;
   FILE* f = fopen(path, "w");
   if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot write %s", path);
@@ -2215,6 +2244,8 @@ extern "C" mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* p
           "# mgl bin layout: upper shape lanes_or_threads size (GROUP: log2 slots, BLOCK: "
           "slots) -- move median %.4f ms\n",
           best_ms);
+  fprintf(f, "hub_agg_threads %d\nhub_agg_per_cu %d\nhub_bucket %lld\n", hbest.threads,
+          hbest.per_cu, (long long)hbest.bucket);
   for (int i = 0; i < best.nb; ++i)
     fprintf(f, "%lld %d %d %d\n", best.b[i].upper == INT64_MAX ? -1LL : (long long)best.b[i].upper,
             best.b[i].shape, best.b[i].s, best.b[i].size);

@@ -359,3 +359,28 @@ def test_binned_layout_batched(ctx, mgl, fused):
             assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
             np.testing.assert_array_equal(lb.assign, lp.assign)
         np.testing.assert_array_equal(b.final_assign, p.final_assign)
+
+
+def test_autotune_layout_parity(ctx, mgl, orc, tmp_path):
+    # the autotuner times candidate bin layouts and hub-aggregate shapes on the
+    # graph and writes the fastest as a tuning file; runs under that file match
+    # the oracle
+    g = hub_graph(mgl)
+    tf = str(tmp_path / "tuned.txt")
+    ctx.autotune(g, tf)
+    text = open(tf).read()
+    print(text, flush=True)
+    assert "hub_agg_threads" in text and "hub_bucket" in text
+    assert text.rstrip().endswith("-1 3 0 0")
+    s = ctx.session(g, tuning_file=tf)
+    k = orc.vertex_weights(g.offsets)
+    rng = np.random.default_rng(17)
+    c = (rng.integers(0, g.n, g.n) // 3 * 3).astype(np.int32)
+    s.set_assign(c)
+    a = orc.community_totals(k, c, g.n)
+    for up in (True, False):
+        np.testing.assert_array_equal(s.move(up),
+                                      orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
+    s.close()
+    compare(ctx.louvain(g, tuning_file=tf, keep_levels=True, keep_graphs=True),
+            orc.louvain(g.offsets, g.cols, None))
