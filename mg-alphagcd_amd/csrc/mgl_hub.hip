@@ -3,17 +3,23 @@
 // The reference gives each such vertex one thread block and a hash table of
 // 2*max_degree slots in global memory, filled with atomicCAS/atomicAdd per
 // edge (src/louvain/louvain.cu:810-1030, table allocated at :1262-1263).
-// Here a hub's row is split into chunks and processed in five steps, none of
-// which issues a global atomic per edge:
-//   count    : gather c[u] once per entry (kept in a compact buffer), sum the
-//              entries of the own community (eici, self-loop), histogram the
-//              candidates over P buckets (P ~ degree / 4096) by a hash of the id
-//   scan     : bucket offsets
-//   scatter  : counting-sort the candidates into their buckets
-//   aggregate: one workgroup per bucket sums equal ids in an LDS table
-//              (an HBM table if a bucket is unusually large), evaluates the
-//              gain of each distinct id and keeps the bucket's best
+// Here a hub's row is split into chunks of HUB_CHUNK entries, and its
+// candidates into P buckets (P ~ degree / bucket size) by a hash of the
+// community id. No step issues a global atomic per edge. Segmented form (the
+// default while no hub of the level has more than HUB_SEG_MAX chunks):
+//   count    : one workgroup per chunk gathers the neighbour record {c, a}
+//              once per entry, sums the entries of the own community (eici,
+//              self-loop) and writes the chunk's candidates (ids and totals
+//              apart) into the chunk's region grouped by bucket (counting
+//              sort in LDS), with the chunk's bucket offsets
+//   aggregate: one workgroup per bucket reads the bucket's segment of every
+//              chunk, sums equal ids in an LDS table (an HBM table if a bucket
+//              is unusually large), evaluates the gain of each distinct id
+//              and keeps the bucket's best
 //   finalize : per hub, the best over its buckets, the bi-probing clamp
+// Scatter form (longer hubs): count (bucket histogram, candidates kept in
+// the chunk's region), scan of the bucket counts, scatter into contiguous
+// buckets, aggregate, finalize.
 // Sums are exact in the integer modes and the argmax is a total order, so the
 // result does not depend on the bucket layout or the order of the scatter.
 #pragma clang fp contract(off)
@@ -21,7 +27,7 @@
 #include "mgl_kernels.h"
 #include "mgl_tables.h"
 
-#define HUB_CHUNK 4096
+#define HUB_CHUNK 8192   // 32 entries per thread of the count step
 #define HUB_T 256
 #define HUB_ITEMS (HUB_CHUNK / HUB_T)
 #define HUB_MAXP 1024

@@ -345,39 +345,71 @@ void launch_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out
 // Community-weight update: a[c] -= k, a[c_new] += k for every moved vertex of
 // an active graph (louvain_baseline.cu:1339-1361 form; louvain.cu:1328-1336
 // with 1377-1381 routes the same integers through a zeroed delta buffer).
-template <class KT>
-__global__ __launch_bounds__(256) void k_update(int64_t n, const vid* __restrict__ c,
-                                                const vid* __restrict__ cn,
-                                                const KT* __restrict__ k, KT* a,
-                                                const int32_t* __restrict__ gid, GState* gs,
-                                                const vid* __restrict__ list) {
-  __shared__ unsigned int moved;
-  if (threadIdx.x == 0) moved = 0;
+// A workgroup of UPD_BS threads takes UPD_IPT consecutive strides of UPD_BS
+// vertices and issues all their c / c_new loads before any update. The moved
+// count is summed per workgroup: one global atomic per workgroup, since
+// atomics on one address serialise (about 10 ns each).
+#define UPD_BS 1024
+#define UPD_IPT 16
+
+// Block sum of one count per thread; the total is valid in thread 0.
+template <int BS>
+__device__ __forceinline__ unsigned int block_count(unsigned int x, unsigned int* ws) {
+  x = group_sum<64>(x);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = x;
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const int64_t v = list ? (int64_t)list[i] : i;
-    const vid x = c[v], y = cn[v];
-    if (x != y) {
-      const int g = gid ? gid[v] : 0;
-      if (gs[g].active) {
-        const KT kv = k[v];
-        atom_add(&a[x], -kv);
-        atom_add(&a[y], kv);
-        atomicAdd(&moved, 1u);
-      }
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0 && moved) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)moved);
+  unsigned int s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < BS / 64; ++i) s += ws[i];
+  return s;
 }
 
-// Integer form: the moves of a workgroup are first summed per community in an
-// LDS table (many vertices of a pass join the same few communities, and their
-// global atomics would serialise on one address); each touched community then
-// takes one global atomic per workgroup. Integer sums make the order
-// irrelevant. A community whose probe sequence meets UPD_PROBES occupied slots
-// goes straight to the global atomic.
-#define UPD_SLOTS 1021
+template <class KT>
+__global__ __launch_bounds__(UPD_BS) void k_update(int64_t n, const vid* __restrict__ c,
+                                                   const vid* __restrict__ cn,
+                                                   const KT* __restrict__ k, KT* a,
+                                                   const int32_t* __restrict__ gid, GState* gs,
+                                                   const vid* __restrict__ list) {
+  __shared__ unsigned int ws[UPD_BS / 64];
+  const int64_t i0 = (int64_t)blockIdx.x * UPD_BS * UPD_IPT + threadIdx.x;
+  int64_t v[UPD_IPT];
+  vid x[UPD_IPT], y[UPD_IPT];
+#pragma unroll
+  for (int j = 0; j < UPD_IPT; ++j) {
+    const int64_t i = i0 + (int64_t)j * UPD_BS;
+    v[j] = i < n ? (list ? (int64_t)list[i] : i) : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < UPD_IPT; ++j) {
+    x[j] = v[j] >= 0 ? c[v[j]] : 0;
+    y[j] = v[j] >= 0 ? cn[v[j]] : 0;
+  }
+  const bool act0 = gid ? true : gs[0].active != 0;
+  unsigned int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < UPD_IPT; ++j) {
+    if (x[j] == y[j]) continue;
+    if (!(gid ? gs[gid[v[j]]].active != 0 : act0)) continue;
+    const KT kv = k[v[j]];
+    atom_add(&a[x[j]], -kv);
+    atom_add(&a[y[j]], kv);
+    ++cnt;
+  }
+  const unsigned int tot = block_count<UPD_BS>(cnt, ws);
+  if (threadIdx.x == 0 && tot) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)tot);
+}
+
+// Integer form: the additions to the joined communities of a workgroup's
+// moves are first summed per community in an LDS table (many vertices of a
+// pass join the same few communities, and their global atomics would
+// serialise on one address); each touched community then takes one global
+// atomic per workgroup. The subtractions from the left communities, which are
+// spread over many communities, go straight to global atomics (timed faster
+// than through the table, also when a third of the moved vertices leave the
+// same thousand communities). Integer sums make the order irrelevant. A community
+// whose probe sequence meets UPD_PROBES occupied slots goes straight to the
+// global atomic.
+#define UPD_SLOTS 4093
 #define UPD_PROBES 8
 __device__ __forceinline__ void upd_add(vid* tk, int64_t* tv, int64_t* a, vid key, int64_t x) {
   uint32_t h = any_slot(key, UPD_SLOTS);
@@ -393,48 +425,60 @@ __device__ __forceinline__ void upd_add(vid* tk, int64_t* tv, int64_t* a, vid ke
   atom_add(&a[key], x);
 }
 
-__global__ __launch_bounds__(256) void k_update_int(int64_t n, const vid* __restrict__ c,
-                                                    const vid* __restrict__ cn,
-                                                    const int64_t* __restrict__ k, int64_t* a,
-                                                    const int32_t* __restrict__ gid, GState* gs,
-                                                    const vid* __restrict__ list) {
+__global__ __launch_bounds__(UPD_BS) void k_update_int(int64_t n, const vid* __restrict__ c,
+                                                       const vid* __restrict__ cn,
+                                                       const int64_t* __restrict__ k, int64_t* a,
+                                                       const int32_t* __restrict__ gid, GState* gs,
+                                                       const vid* __restrict__ list) {
   __shared__ vid tk[UPD_SLOTS];
   __shared__ int64_t tv[UPD_SLOTS];
-  __shared__ unsigned int moved;
-  for (int s = threadIdx.x; s < UPD_SLOTS; s += 256) {
+  __shared__ unsigned int ws[UPD_BS / 64];
+  for (int s = threadIdx.x; s < UPD_SLOTS; s += UPD_BS) {
     tk[s] = MGL_EMPTY;
     tv[s] = 0;
   }
-  if (threadIdx.x == 0) moved = 0;
-  __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-    const int64_t v = list ? (int64_t)list[i] : i;
-    const vid x = c[v], y = cn[v];
-    if (x != y) {
-      const int g = gid ? gid[v] : 0;
-      if (gs[g].active) {
-        const int64_t kv = k[v];
-        upd_add(tk, tv, a, x, -kv);
-        upd_add(tk, tv, a, y, kv);
-        atomicAdd(&moved, 1u);
-      }
-    }
+  const int64_t i0 = (int64_t)blockIdx.x * UPD_BS * UPD_IPT + threadIdx.x;
+  int64_t v[UPD_IPT];
+  vid x[UPD_IPT], y[UPD_IPT];
+#pragma unroll
+  for (int j = 0; j < UPD_IPT; ++j) {
+    const int64_t i = i0 + (int64_t)j * UPD_BS;
+    v[j] = i < n ? (list ? (int64_t)list[i] : i) : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < UPD_IPT; ++j) {
+    x[j] = v[j] >= 0 ? c[v[j]] : 0;
+    y[j] = v[j] >= 0 ? cn[v[j]] : 0;
+  }
+  const bool act0 = gid ? true : gs[0].active != 0;  // one graph: its flag, read once
+  __syncthreads();  // table cleared
+  unsigned int cnt = 0;
+#pragma unroll
+  for (int j = 0; j < UPD_IPT; ++j) {
+    if (x[j] == y[j]) continue;
+    if (!(gid ? gs[gid[v[j]]].active != 0 : act0)) continue;
+    const int64_t kv = k[v[j]];
+    atom_add(&a[x[j]], -kv);
+    upd_add(tk, tv, a, y[j], kv);
+    ++cnt;
   }
   __syncthreads();
-  for (int s = threadIdx.x; s < UPD_SLOTS; s += 256)
+  for (int s = threadIdx.x; s < UPD_SLOTS; s += UPD_BS)
     if (tk[s] != MGL_EMPTY && tv[s] != 0) atom_add(&a[tk[s]], tv[s]);
-  if (threadIdx.x == 0 && moved) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)moved);
+  const unsigned int tot = block_count<UPD_BS>(cnt, ws);
+  if (threadIdx.x == 0 && tot) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)tot);
 }
 
 void launch_update(int mode, int64_t n, const vid* c, const vid* cn, const void* k, void* a,
                    const int32_t* gid, GState* gs, hipStream_t st, const vid* list) {
   if (n <= 0) return;
-  const unsigned grid = grid_for(n, 256, 8192);
+  const unsigned grid = (unsigned)((n + (int64_t)UPD_BS * UPD_IPT - 1) / ((int64_t)UPD_BS * UPD_IPT));
   if (mode_is_int(mode))
-    k_update_int<<<grid, 256, 0, st>>>(n, c, cn, (const int64_t*)k, (int64_t*)a, gid, gs, list);
+    k_update_int<<<grid, UPD_BS, 0, st>>>(n, c, cn, (const int64_t*)k, (int64_t*)a, gid, gs,
+                                          list);
   else
-    k_update<double><<<grid, 256, 0, st>>>(n, c, cn, (const double*)k, (double*)a, gid, gs,
-                                           list);
+    k_update<double><<<grid, UPD_BS, 0, st>>>(n, c, cn, (const double*)k, (double*)a, gid, gs,
+                                              list);
 }
 
 // ---------------------------------------------------------------------------
@@ -454,12 +498,12 @@ struct RedAcc<true> {
     const u128 x = (u128)(uint64_t)(a < 0 ? -a : a);
     sa2 += x * x;
   }
-  __device__ void flush(GState* s) {
-    int64_t t = sin;
-    uint64_t lo = (uint64_t)sa2, hi = (uint64_t)(sa2 >> 64);
-    t = group_sum<64>(t);
-    // 128-bit wave sum: sum the halves separately with carries
-    uint64_t lo_sum = lo, hi_sum = hi;
+  // wavefront sums (valid in every lane): sum_in, and sum a^2 as two 64-bit
+  // halves added with carries
+  __device__ void wave_sum(int64_t& t, uint64_t& lo_sum, uint64_t& hi_sum) const {
+    t = group_sum<64>(sin);
+    lo_sum = (uint64_t)sa2;
+    hi_sum = (uint64_t)(sa2 >> 64);
 #pragma unroll
     for (int m = 32; m > 0; m >>= 1) {
       uint64_t olo = (uint64_t)shfl_xor_t((int64_t)lo_sum, m, 64);
@@ -468,12 +512,43 @@ struct RedAcc<true> {
       hi_sum = hi_sum + ohi + (nlo < lo_sum ? 1ull : 0ull);
       lo_sum = nlo;
     }
+  }
+  __device__ void flush(GState* s) {
+    int64_t t;
+    uint64_t lo_sum, hi_sum;
+    wave_sum(t, lo_sum, hi_sum);
     if ((threadIdx.x & 63) == 0) {
       if (t) atomicAdd((unsigned long long*)&s->sin_i, (unsigned long long)t);
       atomic_add_u128(&s->sa2_lo, &s->sa2_hi, ((u128)hi_sum << 64) | lo_sum);
     }
     sin = 0;
     sa2 = 0;
+  }
+  // Workgroup flush (every thread of the BS-thread workgroup calls it): the
+  // wavefront sums are combined in LDS and thread 0 issues the atomics.
+  template <int BS>
+  __device__ void flush_block(GState* s) {
+    __shared__ int64_t wt[BS / 64];
+    __shared__ uint64_t wl[BS / 64], wh[BS / 64];
+    int64_t t;
+    uint64_t lo_sum, hi_sum;
+    wave_sum(t, lo_sum, hi_sum);
+    if ((threadIdx.x & 63) == 0) {
+      wt[threadIdx.x >> 6] = t;
+      wl[threadIdx.x >> 6] = lo_sum;
+      wh[threadIdx.x >> 6] = hi_sum;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t T = 0;
+      u128 S = 0;
+      for (int i = 0; i < BS / 64; ++i) {
+        T += wt[i];
+        S += ((u128)wh[i] << 64) | wl[i];
+      }
+      if (T) atomicAdd((unsigned long long*)&s->sin_i, (unsigned long long)T);
+      atomic_add_u128(&s->sa2_lo, &s->sa2_hi, S);
+    }
   }
 };
 template <>
@@ -488,6 +563,25 @@ struct RedAcc<false> {
     }
     sin = 0;
     sa2 = 0;
+  }
+  template <int BS>
+  __device__ void flush_block(GState* s) {
+    __shared__ double wt[BS / 64], wu[BS / 64];
+    const double t = group_sum<64>(sin), u = group_sum<64>(sa2);
+    if ((threadIdx.x & 63) == 0) {
+      wt[threadIdx.x >> 6] = t;
+      wu[threadIdx.x >> 6] = u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double T = 0, U = 0;
+      for (int i = 0; i < BS / 64; ++i) {
+        T += wt[i];
+        U += wu[i];
+      }
+      atomicAdd(&s->sin_d, T);
+      atomicAdd(&s->sa2_d, U);
+    }
   }
 };
 
@@ -535,26 +629,41 @@ __global__ __launch_bounds__(256) void k_reduce(int64_t n, const KT* __restrict_
   acc.flush(&gs[gcur]);
 }
 
-// One graph: grid-stride partials, one flush per wavefront.
+// One graph: grid-stride partials with RED_UN independent loads per thread and
+// step, one flush per workgroup (atomics on one address serialise).
+#define RED_BS 1024
+#define RED_UN 4
 template <class KT, bool EXACT>
-__global__ __launch_bounds__(256) void k_reduce1(int64_t n, const KT* __restrict__ vin,
-                                                 const KT* __restrict__ a, GState* gs) {
-  if (!gs[0].active) return;
+__global__ __launch_bounds__(RED_BS) void k_reduce1(int64_t n, const KT* __restrict__ vin,
+                                                    const KT* __restrict__ a, GState* gs) {
+  if (!gs[0].active) return;  // uniform over the workgroup
   RedAcc<EXACT> acc;
-  for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256)
-    acc.add(vin ? vin[v] : (KT)0, a[v]);
-  acc.flush(&gs[0]);
+  const int64_t stride = (int64_t)gridDim.x * RED_BS;
+  for (int64_t v0 = (int64_t)blockIdx.x * RED_BS + threadIdx.x; v0 < n; v0 += stride * RED_UN) {
+    KT xv[RED_UN], xa[RED_UN];
+#pragma unroll
+    for (int u = 0; u < RED_UN; ++u) {
+      const int64_t v = v0 + u * stride;
+      xv[u] = (v < n && vin) ? vin[v] : (KT)0;
+      xa[u] = v < n ? a[v] : (KT)0;
+    }
+#pragma unroll
+    for (int u = 0; u < RED_UN; ++u) acc.add(xv[u], xa[u]);
+  }
+  acc.template flush_block<RED_BS>(&gs[0]);
 }
 
 void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const int32_t* gid,
                    GState* gs, hipStream_t st, const vid* list) {
   if (n <= 0) return;
   if (!gid && !list) {
-    const unsigned grid = grid_for(n, 256 * 8, 1024);
+    const unsigned grid = grid_for(n, RED_BS * RED_UN, 256);
     if (mode_is_int(mode))
-      k_reduce1<int64_t, true><<<grid, 256, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a, gs);
+      k_reduce1<int64_t, true><<<grid, RED_BS, 0, st>>>(n, (const int64_t*)vin, (const int64_t*)a,
+                                                        gs);
     else
-      k_reduce1<double, false><<<grid, 256, 0, st>>>(n, (const double*)vin, (const double*)a, gs);
+      k_reduce1<double, false><<<grid, RED_BS, 0, st>>>(n, (const double*)vin, (const double*)a,
+                                                        gs);
     return;
   }
   const int64_t per_wave = 256;
