@@ -243,8 +243,11 @@ int mgl_device_lds_bytes(const mgl_ctx* ctx);
 /* Copies `bytes` between two device buffers `iters` times with a streaming
  * copy kernel and returns the best read+write GB/s (HBM calibration). */
 mgl_status mgl_membench(mgl_ctx* ctx, int64_t bytes, int iters, double* gbps);
-/* Times the move pass of g under candidate bin layouts and writes the fastest
- * to path (JSON lines). */
+/* Times the move pass of g under candidate bin layouts and hub-path shapes and
+ * writes the fastest to path as a tuning file (text: '#' comment lines,
+ * "hub_agg_threads N", "hub_agg_per_cu N", "hub_bucket N", then one bin per
+ * line, "upper shape lanes_or_threads size"). A malformed tuning file makes
+ * the call that names it fail with MGL_INVALID_ARG. */
 mgl_status mgl_autotune(mgl_ctx* ctx, const mgl_csr* g, const char* path);
 
 #ifdef __cplusplus

@@ -2053,51 +2053,52 @@ mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
-// Tuning file: one line per bin, "upper shape s lg" (text). Written by
-// mgl_autotune, read at context use.
+// Tuning file (text, written by mgl_autotune): lines starting with '#' and
+// empty lines are skipped; "hub_bucket N", "hub_bucket_unclamped N",
+// "hub_segmented 0|1", "hub_agg_threads 256|512|1024" and "hub_agg_per_cu 1..8"
+// set the hub path; every other line is one bin, "upper shape s size", in
+// increasing order of upper (-1: unbounded). Anything else is an error.
 static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
   FILE* f = fopen(path, "r");
   if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot open tuning file %s", path);
   BinLayout L;
   L.nb = 0;
   char line[256];
-  while (fgets(line, sizeof(line), f) && L.nb < MGL_MAX_BINS) {
-    if (line[0] == '#' || line[0] == '\n') continue;
+  const char* bad = nullptr;  // what is wrong, if anything
+  int lineno = 0;
+  while (!bad && fgets(line, sizeof(line), f)) {
+    ++lineno;
+    const char* p = line;
+    while (*p == ' ' || *p == '\t' || *p == '\r') ++p;
+    if (*p == '#' || *p == '\n' || *p == 0) continue;
     long long hb;
-    if (sscanf(line, "hub_bucket %lld", &hb) == 1) {
-      if (hb < 1) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_bucket", path);
+    int hs;
+    char extra;
+    if (sscanf(p, "hub_bucket %lld %c", &hb, &extra) == 1) {
+      if (hb < 1) bad = "hub_bucket";
       tu.hub_bucket = hb;
-      continue;
-    }
-    if (sscanf(line, "hub_bucket_unclamped %lld", &hb) == 1) {
-      if (hb < 1) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_bucket", path);
+    } else if (sscanf(p, "hub_bucket_unclamped %lld %c", &hb, &extra) == 1) {
+      if (hb < 1) bad = "hub_bucket_unclamped";
       tu.hub_bucket = hb;
       tu.hub_bucket_clamp = false;
-      continue;
-    }
-    int hs;
-    if (sscanf(line, "hub_segmented %d", &hs) == 1) {
+    } else if (sscanf(p, "hub_segmented %d %c", &hs, &extra) == 1) {
       tu.hub_segmented = hs != 0;
-      continue;
-    }
-    if (sscanf(line, "hub_agg_threads %d", &hs) == 1) {
-      if (hs != 256 && hs != 512 && hs != 1024)
-        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_agg_threads", path);
+    } else if (sscanf(p, "hub_agg_threads %d %c", &hs, &extra) == 1) {
+      if (hs != 256 && hs != 512 && hs != 1024) bad = "hub_agg_threads";
       tu.hub_agg_threads = hs;
-      continue;
-    }
-    if (sscanf(line, "hub_agg_per_cu %d", &hs) == 1) {
-      if (hs < 1 || hs > 8) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: hub_agg_per_cu", path);
+    } else if (sscanf(p, "hub_agg_per_cu %d %c", &hs, &extra) == 1) {
+      if (hs < 1 || hs > 8) bad = "hub_agg_per_cu";
       tu.hub_agg_per_cu = hs;
-      continue;
-    }
-    long long up;
-    int sh, s, sz;
-    if (sscanf(line, "%lld %d %d %d", &up, &sh, &s, &sz) == 4) {
-      L.b[L.nb++] = {up < 0 ? INT64_MAX : (int64_t)up, sh, s, sz};
+    } else {
+      long long up;
+      int sh, sv, sz;
+      if (sscanf(p, "%lld %d %d %d %c", &up, &sh, &sv, &sz, &extra) != 4) bad = "unknown line";
+      else if (L.nb == MGL_MAX_BINS) bad = "more bins than MGL_MAX_BINS";
+      else L.b[L.nb++] = {up < 0 ? INT64_MAX : (int64_t)up, sh, sv, sz};
     }
   }
   fclose(f);
+  if (bad) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s, line %d: %s", path, lineno, bad);
   if (L.nb < 2 || L.b[0].upper != 0 || L.b[0].shape != SHAPE_NONE)
     FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin 0 must be degree 0", path);
   for (int i = 1; i < L.nb; ++i) {
@@ -2117,7 +2118,8 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
     }
   }
   if (L.b[L.nb - 1].shape != SHAPE_GLOBAL)
-    FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: last bin must use the HBM hash", path);
+    FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: the last bin must be the hub path (shape %d)",
+         path, (int)SHAPE_GLOBAL);
   tu.bins = L;
   tu.have_bins = true;
   return MGL_OK;

@@ -624,10 +624,14 @@ __global__ __launch_bounds__(BS) void k_hub_aggregate_seg(MoveArgs A, HubPlan H,
 }
 
 // ---- finalize -----------------------------------------------------------------
+// One wavefront per hub: its lanes take the hub's buckets in turn, then a
+// wavefront argmax (a total order, so the result does not depend on how the
+// buckets are split over the lanes) and the bi-probing clamp.
 template <class M, class AT>
-__global__ void k_hub_final(MoveArgs A, HubPlan H) {
+__global__ __launch_bounds__(256) void k_hub_final(MoveArgs A, HubPlan H) {
   typedef typename M::K KT;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;  // uniform per wavefront
+  const int lane = threadIdx.x & 63;
   if (i >= H.m) return;
   const vid v = H.hub_v[i];
   const int g = A.gid ? A.gid[v] : 0;
@@ -636,8 +640,10 @@ __global__ void k_hub_final(MoveArgs A, HubPlan H) {
   const bool small = A.tie_small ? true : (A.up != 0);
   double best = 0.0;
   vid dst = ci;
-  for (int64_t b = H.hub_bk[i]; b < H.hub_bk[i + 1]; ++b)
+  for (int64_t b = H.hub_bk[i] + lane; b < H.hub_bk[i + 1]; b += 64)
     if (better(H.bgain[b], H.bdst[b], best, dst, small)) { best = H.bgain[b]; dst = H.bdst[b]; }
+  group_argmax<64>(best, dst, small);
+  if (lane != 0) return;
   A.c_new[v] = (A.version == 1) ? dst : (A.up ? (dst < ci ? dst : ci) : (dst > ci ? dst : ci));
   if (A.vin) ((KT*)A.vin)[v] = ((const KT*)H.hub_eici)[i] + ((const KT*)H.hub_self)[i];
 }
@@ -678,7 +684,7 @@ static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t
     AGG_(k_hub_aggregate, 256) AGG_(k_hub_aggregate, 512) AGG_(k_hub_aggregate, 1024)
   }
 #undef AGG_
-  k_hub_final<M, AT><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
+  k_hub_final<M, AT><<<(unsigned)((H.m * 64 + 255) / 256), 256, 0, st>>>(A, H);
   return 0;
 }
 

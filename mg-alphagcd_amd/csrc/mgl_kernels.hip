@@ -352,18 +352,6 @@ void launch_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out
 #define UPD_BS 1024
 #define UPD_IPT 16
 
-// Block sum of one count per thread; the total is valid in thread 0.
-template <int BS>
-__device__ __forceinline__ unsigned int block_count(unsigned int x, unsigned int* ws) {
-  x = group_sum<64>(x);
-  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = x;
-  __syncthreads();
-  unsigned int s = 0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < BS / 64; ++i) s += ws[i];
-  return s;
-}
-
 template <class KT>
 __global__ __launch_bounds__(UPD_BS) void k_update(int64_t n, const vid* __restrict__ c,
                                                    const vid* __restrict__ cn,
@@ -395,7 +383,7 @@ __global__ __launch_bounds__(UPD_BS) void k_update(int64_t n, const vid* __restr
     atom_add(&a[y[j]], kv);
     ++cnt;
   }
-  const unsigned int tot = block_count<UPD_BS>(cnt, ws);
+  const unsigned int tot = block_sum<UPD_BS, unsigned int>(cnt, ws);
   if (threadIdx.x == 0 && tot) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)tot);
 }
 
@@ -465,7 +453,7 @@ __global__ __launch_bounds__(UPD_BS) void k_update_int(int64_t n, const vid* __r
   __syncthreads();
   for (int s = threadIdx.x; s < UPD_SLOTS; s += UPD_BS)
     if (tk[s] != MGL_EMPTY && tv[s] != 0) atom_add(&a[tk[s]], tv[s]);
-  const unsigned int tot = block_count<UPD_BS>(cnt, ws);
+  const unsigned int tot = block_sum<UPD_BS, unsigned int>(cnt, ws);
   if (threadIdx.x == 0 && tot) atomicAdd((unsigned long long*)&gs[0].moved, (unsigned long long)tot);
 }
 

@@ -193,6 +193,42 @@ def test_tuning_file_applies_to_its_call_only(ctx, mgl, tmp_path):
     s.close()
 
 
+_GOOD_BINS = "0 0 0 0\n32 1 32 6\n-1 3 0 0\n"
+
+
+@pytest.mark.parametrize("text, reason", [
+    ("hub_agg_threads 300\n" + _GOOD_BINS, "hub_agg_threads"),
+    ("hub_agg_per_cu 9\n" + _GOOD_BINS, "hub_agg_per_cu"),
+    ("hub_bucket 0\n" + _GOOD_BINS, "hub_bucket"),
+    ("hub_buckets 3072\n" + _GOOD_BINS, "unknown line"),
+    ("hub_bucket 3072 4096\n" + _GOOD_BINS, "unknown line"),
+    ("0 0 0 0\n32 1 32 6 7\n-1 3 0 0\n", "unknown line"),
+    ("".join(f"{2 * i} 1 4 5\n" for i in range(1, 20)), "more bins"),
+    ("0 0 0 0\n32 1 32 6\n16 1 16 5\n-1 3 0 0\n", "bounds"),
+    ("0 0 0 0\n32 1 32 6\n", "hub path"),
+    ("4 4 1 0\n-1 3 0 0\n", "degree 0"),
+    ("0 0 0 0\n64 1 32 5\n-1 3 0 0\n", "table too small"),
+])
+def test_tuning_file_errors(ctx, mgl, tmp_path, text, reason):
+    # every malformed tuning file is refused with INVALID_ARG naming the
+    # problem; the context stays usable
+    g = mgl.gen_rmat(10, 8, seed=1)
+    tf = tmp_path / "tune.txt"
+    tf.write_text(text)
+    with pytest.raises(mgl.MglError) as e:
+        ctx.session(g, tuning_file=str(tf))
+    assert e.value.code == 1 and reason in str(e.value), str(e.value)
+    with pytest.raises(mgl.MglError) as e:
+        ctx.louvain(g, tuning_file=str(tf))
+    assert e.value.code == 1 and reason in str(e.value), str(e.value)
+    with pytest.raises(mgl.MglError) as e:
+        ctx.session(g, tuning_file=str(tmp_path / "missing.txt"))
+    assert e.value.code == 1
+    s = ctx.session(g)
+    s.pass_()
+    s.close()
+
+
 def test_hub_graph_full_parity(ctx, mgl, orc):
     g = hub_graph(mgl)
     res = ctx.louvain(g, keep_levels=True, keep_graphs=True)
