@@ -233,7 +233,7 @@ struct Engine {
   std::vector<int64_t> bin_n, bin_off;
   // hub plan of the current level (last bin)
   DBuf hchunk, hv, hbk, hbkhub, htmpc, htmpw, hbktc, hbktw, hbcount, hboff, hbcur, heici, hself,
-      hbgain, hbdst, hgk, hgv, hscan, htmpa, hbkta, hga, hchunk0, hcbo0, hcbo, hgalloc;
+      hbgain, hbdst, hgk, hgv, hscan, htmpa, hbkta, hga, hchunk0, hcbo0, hcbo, hgalloc, hbmeta;
   HubPlan hub;
   // coarsening buffers
   DBuf cflag, crank, cnewc, ck0, ck1, cv0, cv1, chist, chead, cuidx, crowcnt, cscan;
@@ -382,7 +382,9 @@ struct Engine {
       HIPC(ctx, rr.ensure(16 * m));
       launch_row_ranges(g.off.as<eid>(), hperm, m, rr.as<int64_t>(), st());
       std::vector<int64_t> ranges(2 * m);
+      std::vector<vid> hubv(m);
       HIPC(ctx, hipMemcpyAsync(ranges.data(), rr.p, 16 * m, hipMemcpyDeviceToHost, st()));
+      HIPC(ctx, hipMemcpyAsync(hubv.data(), hperm, 4 * m, hipMemcpyDeviceToHost, st()));
       HIPC(ctx, hipStreamSynchronize(st()));
       const int64_t CH = hub_chunk_edges();
       // owner tables of the aggregate: (key, sum) slots in a share of the
@@ -403,6 +405,7 @@ struct Engine {
         if (BK < 64) BK = 64;
       }
       std::vector<int64_t> chunk, hbkv(m + 1, 0), bkhub, chunk0(m + 1, 0), cbo0(m, 0);
+      std::vector<HubBucket> bmeta;
       int64_t pos = 0, ncbo = 0, max_nch = 0;
       for (int64_t i = 0; i < m; ++i) {
         const int64_t e0 = ranges[2 * i], e1 = ranges[2 * i + 1], d = e1 - e0;
@@ -412,13 +415,17 @@ struct Engine {
           chunk.push_back(e);
           chunk.push_back(pos + (e - e0));
         }
-        pos += d;
         int64_t P = (d + BK - 1) / BK;
         if (P < 1) P = 1;
         if (P > hub_max_buckets()) P = hub_max_buckets();
         hbkv[i + 1] = hbkv[i] + P;
-        for (int64_t j = 0; j < P; ++j) bkhub.push_back(i);
         const int64_t nch = (int64_t)chunk.size() / 3 - chunk0[i];
+        for (int64_t j = 0; j < P; ++j) {
+          bkhub.push_back(i);
+          bmeta.push_back(HubBucket{pos, ncbo, (int32_t)i, hubv[i], (int32_t)j, (int32_t)P,
+                                    (int32_t)nch, 0});
+        }
+        pos += d;
         cbo0[i] = ncbo;
         ncbo += nch * (P + 1);
         max_nch = std::max(max_nch, nch);
@@ -465,6 +472,9 @@ struct Engine {
       HIPC(ctx, hipMemcpyAsync(hv.p, hperm, 4 * m, hipMemcpyDeviceToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hbk.p, hbkv.data(), 8 * (m + 1), hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hbkhub.p, bkhub.data(), 8 * hub.nbuckets, hipMemcpyHostToDevice, st()));
+      HIPC(ctx, hbmeta.ensure(sizeof(HubBucket) * hub.nbuckets));
+      HIPC(ctx, hipMemcpyAsync(hbmeta.p, bmeta.data(), sizeof(HubBucket) * hub.nbuckets,
+                               hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hchunk0.p, chunk0.data(), 8 * (m + 1), hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hcbo0.p, cbo0.data(), 8 * m, hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipStreamSynchronize(st()));
@@ -493,6 +503,7 @@ struct Engine {
       hub.hub_cbo0 = hcbo0.as<int64_t>();
       hub.cbo = hcbo.as<int32_t>();
       hub.galloc = hgalloc.as<unsigned long long>();
+      hub.bmeta = hbmeta.as<HubBucket>();
       hub.lds_slots = Hlds;
       hub.agg_bs = tune.hub_agg_threads;
     }

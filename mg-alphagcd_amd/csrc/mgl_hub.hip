@@ -583,17 +583,18 @@ __global__ __launch_bounds__(BS) void k_hub_aggregate_seg(MoveArgs A, HubPlan H,
   __shared__ int32_t srcs[HUB_SEG_MAX];
   __shared__ int32_t wsum[BS / 64];
   const int64_t b = blockIdx.x;
-  const int64_t i = H.bk_hub[b];
-  const vid v = H.hub_v[i];
+  const HubBucket hb = H.bmeta[b];  // one (scalar) load: the bucket's fixed layout
+  const int64_t i = hb.hub;
+  const vid v = hb.v;
   const int g = A.gid ? A.gid[v] : 0;
   if (!A.gs[g].active) return;
-  const int64_t lb = b - H.hub_bk[i];
-  const int64_t P = H.hub_bk[i + 1] - H.hub_bk[i];
-  const int64_t c0 = H.hub_chunk0[i];
-  const int nch = (int)(H.hub_chunk0[i + 1] - c0);
-  const int64_t cb0 = H.hub_cbo0[i];
-  const int64_t base = H.chunk[3 * c0 + 2];
-  // segment lengths, CPT consecutive chunks per thread (nch <= HUB_SEG_MAX)
+  const int64_t lb = hb.lb;
+  const int64_t P = hb.P;
+  const int nch = hb.nch;
+  const int64_t cb0 = hb.cb0;
+  const int64_t base = hb.base;
+  // segment lengths, CPT consecutive chunks per thread (nch <= HUB_SEG_MAX);
+  // chunk c of the hub starts at base + c * HUB_CHUNK in tmp
   constexpr int CPT = (HUB_SEG_MAX + BS - 1) / BS;
   int32_t len[CPT];
   int32_t tot = 0;
@@ -605,7 +606,7 @@ __global__ __launch_bounds__(BS) void k_hub_aggregate_seg(MoveArgs A, HubPlan H,
       const int64_t q = cb0 + (int64_t)c * (P + 1) + lb;
       const int32_t s = H.cbo[q];
       len[k] = H.cbo[q + 1] - s;
-      srcs[c] = (int32_t)(H.chunk[3 * (c0 + c) + 2] - base) + s;
+      srcs[c] = c * HUB_CHUNK + s;
     }
     tot += len[k];
   }

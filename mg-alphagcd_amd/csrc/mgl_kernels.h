@@ -87,6 +87,19 @@ void launch_remap_cols(const vid* col, int64_t nnz, const vid* rpos, vid* colr, 
 void move_kernels_allow_lds(int bytes);
 
 // ---- hub path (mgl_hub.hip) ------------------------------------------------
+// One bucket of the segmented hub path, fixed for a level (built on the host
+// with the plan): its hub, and where that hub's candidates and bucket offsets
+// are. Chunk c of the hub starts at tmp position base + c * hub_chunk_edges().
+struct HubBucket {
+  int64_t base;  // tmp position of the hub's first chunk
+  int64_t cb0;   // first cbo entry of the hub
+  int32_t hub;   // hub index
+  vid v;         // hub vertex
+  int32_t lb;    // bucket index within the hub
+  int32_t P;     // buckets of the hub
+  int32_t nch;   // chunks of the hub
+  int32_t pad;
+};
 struct HubPlan {
   int64_t m = 0, nchunks = 0, nbuckets = 0, ehub = 0;
   const int64_t* chunk = nullptr;   // [3*nchunks]: hub index, first CSR entry, compact position
@@ -120,6 +133,7 @@ struct HubPlan {
   const int64_t* hub_cbo0 = nullptr;    // [m] first cbo entry of each hub
   int32_t* cbo = nullptr;               // [sum over hubs of chunks x (P_i + 1)]
   unsigned long long* galloc = nullptr; // [1] HBM-table slots handed out in this pass
+  const HubBucket* bmeta = nullptr;     // [nbuckets] (segmented form)
   int agg_bs = 1024;                    // threads per aggregate workgroup: 256, 512 or 1024
 };
 int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStream_t st);
