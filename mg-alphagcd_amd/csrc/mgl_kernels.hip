@@ -66,10 +66,59 @@ __global__ __launch_bounds__(256) void k_build_rec(int64_t first, int64_t n,
   }
 }
 
+// Degree-bin record layout built in vertex order: c[v] and a[c[v]] are read
+// in order (a community id is mostly its member's own id or near it) and each
+// record goes to rpos[v]. Consecutive vertices of one bin have consecutive
+// positions, so the stores come in short runs, while the position-order loop
+// above reads c and a once per bin (about seven sweeps of both arrays on
+// R-MAT 24). Records at positions below `first` (isolated vertices) are not
+// built. All loads of a thread are issued before its stores.
+#define REC_IPT 8
+template <class KT, class AT>
+__global__ __launch_bounds__(256) void k_build_rec_vo(int64_t first, int64_t n,
+                                                      const vid* __restrict__ c,
+                                                      const KT* __restrict__ a,
+                                                      Rec<AT>* __restrict__ rec,
+                                                      const vid* __restrict__ rpos) {
+  const int64_t v0 = (int64_t)blockIdx.x * 256 * REC_IPT + threadIdx.x;
+  vid x[REC_IPT];
+  int64_t at[REC_IPT];
+  KT av[REC_IPT];
+#pragma unroll
+  for (int j = 0; j < REC_IPT; ++j) {
+    const int64_t v = v0 + (int64_t)j * 256;
+    at[j] = v < n ? (int64_t)rpos[v] : -1;
+    x[j] = v < n ? c[v] : 0;
+  }
+#pragma unroll
+  for (int j = 0; j < REC_IPT; ++j) av[j] = at[j] >= first ? a[x[j]] : (KT)0;
+#pragma unroll
+  for (int j = 0; j < REC_IPT; ++j) {
+    if (at[j] < first) continue;  // beyond n (at = -1) or an isolated vertex
+    Rec<AT> r;
+    r.c = x[j];
+    r.a = (AT)av[j];
+    rec[at[j]] = r;
+  }
+}
+
 void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* a, void* rec,
                       hipStream_t st, const vid* list, const vid* rpos, const vid* rinv,
                       int64_t first) {
   if (n - first <= 0) return;
+  if (!list && rpos && rinv) {
+    const unsigned gv = (unsigned)((n + 256 * REC_IPT - 1) / (256 * REC_IPT));
+    if ((mode == MODE_UNIT || mode == MODE_I32) && rec32)
+      k_build_rec_vo<int64_t, int32_t><<<gv, 256, 0, st>>>(first, n, c, (const int64_t*)a,
+                                                           (Rec<int32_t>*)rec, rpos);
+    else if (mode_is_int(mode))
+      k_build_rec_vo<int64_t, int64_t><<<gv, 256, 0, st>>>(first, n, c, (const int64_t*)a,
+                                                           (Rec<int64_t>*)rec, rpos);
+    else
+      k_build_rec_vo<double, double><<<gv, 256, 0, st>>>(first, n, c, (const double*)a,
+                                                         (Rec<double>*)rec, rpos);
+    return;
+  }
   const unsigned grid = grid_for(n - first, 256, 8192);
   if ((mode == MODE_UNIT || mode == MODE_I32) && rec32)
     k_build_rec<int64_t, int32_t><<<grid, 256, 0, st>>>(first, n, c, (const int64_t*)a,
