@@ -119,6 +119,36 @@ def test_hub_path_star(ctx, mgl, orc):
     compare(res, ref)
 
 
+def test_hub_path_giant_star(ctx, mgl, orc):
+    # a star with 4.4 M leaves plus a ring: the centre's row has more chunks
+    # than the segmented hub form takes (so the scatter form runs), its 1,024
+    # buckets hold about 4,300 distinct singletons each in the first pass,
+    # beyond the owner table and the scanned LDS table, so the aggregate falls
+    # back to its HBM table
+    n = 4_400_001
+    leaves = np.arange(1, n, dtype=np.int64)
+    u = np.concatenate([np.zeros(n - 1, dtype=np.int64), leaves[:-1]])
+    v = np.concatenate([leaves, leaves[1:]])
+    g = mgl.CSR.from_edges(n, np.stack([u, v], axis=1))
+    s = ctx.session(g)
+    assert s.bins()[-1] == 1
+    # the centre's proposal under singletons (all candidates distinct), random
+    # communities and coarse groups (many duplicates per bucket)
+    k = orc.vertex_weights(g.offsets)
+    rng = np.random.default_rng(11)
+    for c in (np.arange(n, dtype=np.int32), rng.integers(0, n, n).astype(np.int32),
+              (np.arange(n, dtype=np.int32) // 1000) * 1000):
+        s.set_assign(c)
+        a = orc.community_totals(k, c, n)
+        for up in (True, False):
+            np.testing.assert_array_equal(
+                s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
+    s.close()
+    res = ctx.louvain(g, keep_levels=True, keep_graphs=True)
+    ref = orc.louvain(g.offsets, g.cols, None)
+    compare(res, ref)
+
+
 def hub_graph(mgl):
     # R-MAT with a heavy tail: many rows in the hub bin
     return mgl.gen_rmat(17, 48, seed=13)
