@@ -10,6 +10,9 @@ rmat (default): R-MAT scale 24, edge factor 32, (a,b,c,d) = (0.57, 0.19, 0.19,
   community-weight update, modularity, commit-or-reject, with the graph
   resident in HBM. W warm-up passes run from the singleton state, then K passes
   are timed. value = directed CSR entries x K / wall time of the K passes.
+  When a timed pass rejects its proposal (the level ends after 17 passes on
+  this graph), the clock stops, the session restarts from singletons and the
+  clock resumes: every timed step is one whole pass and nothing else.
 
 batched: 4,096 independent graphs, vertex counts log-uniform in [1,000,
   200,000] (seed 4,000,000 + i); even i planted partition (8-256 communities,
@@ -34,7 +37,10 @@ c_new) + distinct (vertex, community) pairs x 8 (a[c_j]). traffic = counted
 HBM bytes per launch of that kernel from the committed rocprofv3 profile.
 
 cpu_baseline: the CPU oracle (C++/OpenMP restatement of the same rule) on a
-bounded sample of the same workload on the host cores.
+bounded sample of the same workload on the host cores: one warm-up pass from
+singletons, then the median of the next 5 passes (batched: full runs of the
+first graphs). The host's logical CPU count, OMP_NUM_THREADS and CPU model are
+recorded with it.
 """
 import argparse
 import ctypes as C
@@ -89,10 +95,20 @@ def _p(x, t):
     return x.ctypes.data_as(C.POINTER(t))
 
 
+def host_info(orc):
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"cores": int(orc.lib.orc_num_threads()), "nproc": usable,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": cpu_model()}
+
+
 # ---------------------------------------------------------------------------
 # CPU baselines (the oracle on the host cores)
 def cpu_baseline_pass(g, passes):
-    """The oracle timing `passes` level-0 passes from singletons (exact mode)."""
+    """The oracle on level-0 passes from singletons (exact mode): pass 1 is a
+    warm-up, then the median time of the next `passes` passes."""
     orc = load_oracle()
     n = g.n
     k = orc.vertex_weights(g.offsets)
@@ -100,17 +116,20 @@ def cpu_baseline_pass(g, passes):
     a = k.copy()
     cn = np.empty(n, np.int32)
     q = -1.0
-    t0 = time.perf_counter()
-    for i in range(passes):
+    times = []
+    for i in range(passes + 1):
+        t0 = time.perf_counter()
         q = orc.lib.orc_pass_i64(n, _p(g.offsets, C.c_int64), _p(g.cols, C.c_int32), None,
                                  _p(k, C.c_int64), _p(c, C.c_int32), _p(a, C.c_int64), g.nnz,
                                  1 if i % 2 == 0 else 0, 2, _p(cn, C.c_int32), q, 1e-6)
-    dt = time.perf_counter() - t0
-    return dict(value=g.nnz * passes / dt, unit="edges/s", cores=int(orc.lib.orc_num_threads()),
-                kind="port",
-                sample=f"{passes} level-0 pass(es) from singletons of the same graph "
-                       f"({g.nnz} entries), CPU oracle C++/OpenMP, {cpu_model()}, "
-                       f"{dt:.2f} s")
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times[1:]))
+    out = dict(value=g.nnz / med, unit="edges/s", kind="port",
+               sample=f"level-0 passes of the same graph ({g.nnz} entries) from singletons: "
+                      f"pass 1 warm-up, median of passes 2-{passes + 1} "
+                      f"({', '.join(f'{t:.2f}' for t in times[1:])} s), CPU oracle C++/OpenMP")
+    out.update(host_info(orc))
+    return out
 
 
 def cpu_baseline_pass_f64(g, passes):
@@ -133,17 +152,20 @@ def cpu_baseline_pass_f64(g, passes):
     a = k.copy()
     cn = np.empty(n, np.int32)
     q = -1.0
-    t0 = time.perf_counter()
-    for i in range(passes):
+    times = []
+    for i in range(passes + 1):
+        t0 = time.perf_counter()
         q = f(n, _p(g.offsets, C.c_int64), _p(g.cols, C.c_int32), _p(w, C.c_double),
               _p(k, C.c_double), _p(c, C.c_int32), _p(a, C.c_double), mass,
               1 if i % 2 == 0 else 0, 2, _p(cn, C.c_int32), q, 1e-6)
-    dt = time.perf_counter() - t0
-    return dict(value=g.nnz * passes / dt, unit="edges/s", cores=int(orc.lib.orc_num_threads()),
-                kind="port",
-                sample=f"{passes} FP64 level-0 pass(es) from singletons of the same graph "
-                       f"({g.nnz} entries), CPU oracle C++/OpenMP, {cpu_model()}, "
-                       f"{dt:.2f} s")
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times[1:]))
+    out = dict(value=g.nnz / med, unit="edges/s", kind="port",
+               sample=f"FP64 level-0 passes of the same graph ({g.nnz} entries) from "
+                      f"singletons: pass 1 warm-up, median of passes 2-{passes + 1} "
+                      f"({', '.join(f'{t:.2f}' for t in times[1:])} s), CPU oracle C++/OpenMP")
+    out.update(host_info(orc))
+    return out
 
 
 def cpu_baseline_batched(graphs, budget_s):
@@ -159,11 +181,12 @@ def cpu_baseline_batched(graphs, budget_s):
         if time.perf_counter() - t0 > budget_s:
             break
     dt = time.perf_counter() - t0
-    return dict(value=done / dt, unit="graphs/s", cores=int(orc.lib.orc_num_threads()),
-                kind="port",
-                sample=f"full multi-level runs of the first {done} graphs of the workload "
-                       f"({sum(g.n for g in graphs[:done])} vertices), CPU oracle C++/OpenMP "
-                       f"exact mode, {cpu_model()}, {dt:.2f} s")
+    out = dict(value=done / dt, unit="graphs/s", kind="port",
+               sample=f"full multi-level runs of the first {done} graphs of the workload "
+                      f"({sum(g.n for g in graphs[:done])} vertices), CPU oracle C++/OpenMP "
+                      f"exact mode, {dt:.2f} s")
+    out.update(host_info(orc))
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -181,32 +204,41 @@ def traffic_for(workload, kernel):
 
 
 def timed_passes(sess, steps, warmup):
-    """W warm-up passes from the session's reset state, then K passes timed as
-    one block: the wall clock runs between two points where the device is idle
-    (every pass_() ends with a stream synchronisation, as the reference's host
-    reads Q after each pass). The counters of each pass's move (distinct
-    (vertex, community) pairs, LDS probes; global atomics, so never inside a
-    timed pass) come from a replay: reset, the same warm-up, then count() before
-    each pass. The replay follows the same trajectory (checked on Q; in the
-    exact modes it is bit-identical)."""
+    """W warm-up passes from the session's reset state, then K passes timed:
+    the wall clock runs between two points where the device is idle (every
+    pass_() ends with a stream synchronisation, as the reference's host reads Q
+    after each pass). A pass that rejects its proposal ends the level; if
+    timed passes remain, the clock stops, the session restarts from singletons
+    (mgl_session_reset) and the clock resumes, so the timed region holds the K
+    passes only. The counters of each pass's move (distinct (vertex,
+    community) pairs, LDS probes; global atomics, so never inside a timed pass)
+    come from a replay: reset, the same warm-up and restarts, then count()
+    before each pass. The replay follows the same trajectory (checked on Q; in
+    the exact modes it is bit-identical)."""
+    restarts = []
+
     def warm():
         for i in range(warmup):
             q, acc = sess.pass_()
             log(f"warmup pass {i + 1}: Q={q:.9f} accepted={acc} "
                 f"{sess.stats()['pass_ms']:.3f} ms")
+            if not acc:
+                sess.reset()
     warm()
     stats = []
+    wall = 0.0
     t0 = time.perf_counter()
-    rejected = 0
     for i in range(steps):
         _, acc = sess.pass_()
-        rejected += 0 if acc else 1
         stats.append(sess.stats())
-    wall = time.perf_counter() - t0
-    if rejected:
-        # a rejected proposal ends the level; the session then rebuilds the
-        # community totals inside that step
-        log(f"note: {rejected} timed pass(es) were rejected (level end); use fewer --steps")
+        if not acc and i + 1 < steps:
+            wall += time.perf_counter() - t0
+            restarts.append(i)
+            sess.reset()
+            t0 = time.perf_counter()
+    wall += time.perf_counter() - t0
+    if restarts:
+        log(f"level restarts (untimed) after timed passes {[warmup + i + 1 for i in restarts]}")
     # replay with the counters
     sess.reset()
     warm()
@@ -216,6 +248,8 @@ def timed_passes(sess, steps, warmup):
         sess.count()
         sess.pass_()
         c = sess.stats()
+        if i in restarts:
+            sess.reset()
         if c["q"] != stats[i]["q"]:
             log(f"replay pass {warmup + i + 1}: Q {c['q']!r} differs from the timed "
                 f"{stats[i]['q']!r}")
@@ -248,6 +282,7 @@ def pass_report(stats, wall, steps, nnz, weight_bytes, workload_tag):
     ins = sum(s["lds_inserts"] for s in stats)
     prb = sum(s["lds_probes"] for s in stats)
     mul = sum(s["lds_multi_probe"] for s in stats)
+    pass_ms = [s["pass_ms"] for s in stats]
     roof = {
         "bound": "hbm",
         "kernel": dom,
@@ -262,6 +297,8 @@ def pass_report(stats, wall, steps, nnz, weight_bytes, workload_tag):
     if traffic:
         roof["traffic_GBps"] = traffic / (dom_ms * 1e-3) / 1e9
     extra = {
+        "pass_ms_median": float(np.median(pass_ms)),
+        "pass_ms_device": pass_ms,
         "pass_algorithmic_GBps": pass_bytes / (ms_per_step * 1e-3) / 1e9,
         "move_ms": float(np.mean([s["move_ms"] for s in stats])),
         "hub_path": {"ms": float(np.mean([s["hub_ms"] for s in stats])),
@@ -357,17 +394,7 @@ def run_rmat(args, mgl):
 
 # ---------------------------------------------------------------------------
 def make_batch_graph(mgl, i):
-    rng = np.random.default_rng(4_000_000 + i)
-    n = int(round(math.exp(rng.uniform(math.log(1_000), math.log(200_000)))))
-    seed = 4_000_000 + i
-    if i % 2 == 0:
-        k = int(rng.integers(8, 257))
-        k = max(1, min(k, n // 8))
-        return mgl.gen_sbm(n, k, exponent=2.1, min_size=8, deg_in=16, deg_out=3, seed=seed,
-                           weights=1)
-    ef = float(rng.uniform(8, 32))
-    scale = max(1, math.ceil(math.log2(n)))
-    return mgl.gen_rmat(scale, ef, 0.57, 0.19, 0.19, seed=seed, weights=1, n_limit=n)
+    return mgl.gen_batch_graph(i)
 
 
 def run_batched(args, mgl):
@@ -555,7 +582,8 @@ def main():
     ap.add_argument("--no-fuse", action="store_true")
     ap.add_argument("--record-layout", type=int, default=0,
                     help="0 auto, 1 vertex order, 2 degree-bin order")
-    ap.add_argument("--cpu-passes", type=int, default=1)
+    ap.add_argument("--cpu-passes", type=int, default=5,
+                    help="rmat/mixed: CPU-baseline passes timed after one warm-up pass")
     ap.add_argument("--cpu-seconds", type=float, default=20.0,
                     help="batched: CPU-baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
@@ -576,9 +604,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design note, not a code path)", file=sys.stderr)
         sys.exit(2)
-    # level-0 passes of the R-MAT workloads: 2 warm-up + K timed passes stay inside
-    # the level (scale 24 ef 32: 17 iterations; scale 25 ef 16 lognormal: 11, the
-    # last one rejected), so no timed pass ends the level
+    # level-0 passes of the R-MAT workloads (scale 24 ef 32: 17 iterations;
+    # scale 25 ef 16 lognormal: 11, the last one rejected); a timed pass that
+    # ends the level is followed by an untimed restart
     defaults = {"rmat": (24, 32, 3, 10), "mixed": (25, 16, 5, 8), "batched": (0, 0, 0, 4)}
     sc, ef, sd, st = defaults[args.workload]
     args.scale = sc if args.scale is None else args.scale

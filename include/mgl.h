@@ -50,13 +50,18 @@ enum {
   MGL_TIE_SMALLEST = 1   /* always the smallest id */
 };
 
-/* Host CSR. Directed entries; an undirected edge appears in both rows. */
+/* Host CSR. Directed entries; an undirected edge appears in both rows.
+ * Replaces HostGraph's arrays (include/graph/host_graph.h:21-43: uint32
+ * offsets and columns, weight_t = double weights, include/common.h:85-87).
+ * Weights come as doubles (w64, the reference's weight_t), as floats (w), or
+ * not at all (unit weights); w64 takes precedence when both are set. */
 typedef struct {
   int64_t n;              /* vertices */
   int64_t nnz;            /* directed CSR entries = offsets[n] */
   const int64_t* offsets; /* [n+1], offsets[0] = 0, non-decreasing */
   const int32_t* cols;    /* [nnz], 0 <= col < n */
-  const float* w;         /* [nnz] or NULL for unit weights */
+  const float* w;         /* [nnz] or NULL */
+  const double* w64;      /* [nnz] or NULL */
 } mgl_csr;
 
 /* Options; MG_GCD flags -threshold, -max_phases, -max_iter, -version
@@ -115,6 +120,13 @@ int64_t mgl_result_level_n(const mgl_result* r, int level);
 double mgl_result_level_q(const mgl_result* r, int level);
 int mgl_result_level_iters(const mgl_result* r, int level);
 const double* mgl_result_level_qtraj(const mgl_result* r, int level); /* [iters] */
+/* Modularity of the level's singleton start: the reference's "Loop 0" row
+ * (louvain.cu:1509-1538). */
+double mgl_result_level_q0(const mgl_result* r, int level);
+/* Device time of each inner iteration of the level in seconds [iters]: the
+ * reference's per-loop time column (louvain.cu:1650-1655), taken from the
+ * device clock at each decision (in a batch, the iteration of the union). */
+const double* mgl_result_level_iter_seconds(const mgl_result* r, int level);
 const int32_t* mgl_result_level_assign(const mgl_result* r, int level); /* NULL unless keep_levels */
 /* Coarse graph that is the INPUT of `level` (level >= 1); weights as doubles in
  * the graph's units (exact mode: integers or dyadic values). NULL unless keep_graphs. */
@@ -125,6 +137,7 @@ mgl_status mgl_result_level_graph(const mgl_result* r, int level, int64_t* n, in
  * the time of the level's end (bookkeeping and coarsening). */
 double mgl_result_level_seconds(const mgl_result* r, int level, double* coarsen_seconds);
 const int32_t* mgl_result_final_assign(const mgl_result* r); /* dense labels of level-0 vertices */
+int64_t mgl_result_vertices(const mgl_result* r); /* level-0 vertices (length of the final assignment) */
 double mgl_result_final_q(const mgl_result* r);
 int mgl_result_precision(const mgl_result* r); /* precision actually used */
 double mgl_result_seconds(const mgl_result* r); /* device wall time of the run */
@@ -182,6 +195,17 @@ mgl_status mgl_session_set_assign(mgl_session* s, const int32_t* c);
 mgl_status mgl_session_move(mgl_session* s, int up, int32_t* c_new);
 /* Replaces compute_modularity (louvain.cu:105-238) for the committed c. */
 mgl_status mgl_session_modularity(mgl_session* s, double* q);
+/* Replaces compute_modularity (louvain.cu:105-238) for any partition of a
+ * graph that is already in device memory: d_offsets [n+1], d_cols [nnz],
+ * d_w [nnz] float weights or NULL (unit weights: exact integer sums, Q bit-equal
+ * to the exact modes'), d_assign [n] community ids in [0, n). Device pointers
+ * of the context's device; runs on the context's stream and returns once *q
+ * is set. Out-of-range ids, columns or rows are skipped on the device and
+ * reported as MGL_INVALID_GRAPH. The community totals and sums live in device
+ * memory this call allocates and frees. */
+mgl_status mgl_modularity_dev(mgl_ctx* ctx, int64_t n, int64_t nnz, const int64_t* d_offsets,
+                              const int32_t* d_cols, const float* d_w, const int32_t* d_assign,
+                              double* q);
 /* Community totals a[n] (as doubles, in graph units) of the committed state. */
 mgl_status mgl_session_get_weights(mgl_session* s, double* a, double* k);
 /* Replaces compute_community_weight(_local_atomic) (louvain.cu:1299-1396,
@@ -200,7 +224,8 @@ typedef struct {
   int64_t n, nnz;
   int64_t* offsets;
   int32_t* cols;
-  float* w; /* NULL for unit weights */
+  float* w;     /* NULL for unit weights (generators: float weights) */
+  double* w64;  /* NULL, or the weights as parsed doubles (mgl_read_mtx) */
 } mgl_host_csr;
 /* weights: 0 unit, 1 uniform (0.5,1.5] (1.5 - U[0,1) in float), 2 lognormal(0,1) as float */
 mgl_status mgl_gen_rmat(int scale, double edge_factor, double a, double b, double c,
@@ -213,7 +238,8 @@ mgl_status mgl_gen_sbm(int64_t n, int64_t n_comm, double exponent, int64_t min_s
 /* Reads a Matrix Market file with the reference loader's semantics
  * (src/mmio/mmio_wrapper.cu:76-295): "symmetric" banner = unit weights, every
  * off-diagonal entry mirrored, diagonal kept once; "general" banner = only
- * entries with col < row kept, weights parsed (1.0 if absent), mirrored.
+ * entries with col < row kept, weights parsed as doubles (1.0 if absent; the
+ * reference's atof into weight_t, :243-250) into w64, mirrored; w is NULL.
  * Rows are sorted by column (the reference's order within a row is not fixed). */
 mgl_status mgl_read_mtx(const char* path, mgl_host_csr* out);
 /* Writes assignment / CSR files (text). */

@@ -49,12 +49,13 @@ class MglError(RuntimeError):
 
 class _Csr(C.Structure):
     _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("offsets", C.c_void_p),
-                ("cols", C.c_void_p), ("w", C.c_void_p)]
+                ("cols", C.c_void_p), ("w", C.c_void_p), ("w64", C.c_void_p)]
 
 
 class _HostCsr(C.Structure):
     _fields_ = [("n", C.c_int64), ("nnz", C.c_int64), ("offsets", C.POINTER(C.c_int64)),
-                ("cols", C.POINTER(C.c_int32)), ("w", C.POINTER(C.c_float))]
+                ("cols", C.POINTER(C.c_int32)), ("w", C.POINTER(C.c_float)),
+                ("w64", C.POINTER(C.c_double))]
 
 
 class _Opts(C.Structure):
@@ -104,11 +105,14 @@ _sig("mgl_result_level_n", _i64, [_vp, _i])
 _sig("mgl_result_level_q", _d, [_vp, _i])
 _sig("mgl_result_level_iters", _i, [_vp, _i])
 _sig("mgl_result_level_qtraj", C.POINTER(_d), [_vp, _i])
+_sig("mgl_result_level_q0", _d, [_vp, _i])
+_sig("mgl_result_level_iter_seconds", C.POINTER(_d), [_vp, _i])
 _sig("mgl_result_level_assign", C.POINTER(_i32), [_vp, _i])
 _sig("mgl_result_level_graph", _i, [_vp, _i, C.POINTER(_i64), C.POINTER(_i64),
                                      C.POINTER(C.POINTER(_i64)), C.POINTER(C.POINTER(_i32)),
                                      C.POINTER(C.POINTER(_d))])
 _sig("mgl_result_final_assign", C.POINTER(_i32), [_vp])
+_sig("mgl_result_vertices", _i64, [_vp])
 _sig("mgl_result_level_seconds", _d, [_vp, _i, C.POINTER(_d)])
 _sig("mgl_result_final_q", _d, [_vp])
 _sig("mgl_result_precision", _i, [_vp])
@@ -139,6 +143,7 @@ _sig("mgl_bin_bounds", _i, [_vp, C.POINTER(_i64), _i])
 _sig("mgl_autotune", _i, [_vp, C.POINTER(_Csr), C.c_char_p])
 _sig("mgl_device_lds_bytes", _i, [_vp])
 _sig("mgl_membench", _i, [_vp, _i64, _i, C.POINTER(_d)])
+_sig("mgl_modularity_dev", _i, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, C.POINTER(_d)])
 
 
 def _p(a, t):
@@ -148,16 +153,27 @@ def _p(a, t):
 # ---------------------------------------------------------------------------
 @dataclass
 class CSR:
-    """Host CSR graph: directed entries, an undirected edge in both rows."""
+    """Host CSR graph: directed entries, an undirected edge in both rows.
+    Weights: None (unit), float64 (passed as doubles, the reference's
+    weight_t) or any other dtype, converted to float32."""
     offsets: np.ndarray  # int64 [n+1]
     cols: np.ndarray     # int32 [nnz]
-    w: Optional[np.ndarray] = None  # float32 [nnz] or None (unit weights)
+    w: Optional[np.ndarray] = None  # float32 or float64 [nnz], or None
 
     def __post_init__(self):
         self.offsets = np.ascontiguousarray(self.offsets, dtype=np.int64)
         self.cols = np.ascontiguousarray(self.cols, dtype=np.int32)
         if self.w is not None:
-            self.w = np.ascontiguousarray(self.w, dtype=np.float32)
+            dt = np.float64 if np.asarray(self.w).dtype == np.float64 else np.float32
+            self.w = np.ascontiguousarray(self.w, dtype=dt)
+        # the C ABI reads nnz entries of each array: short arrays are refused here
+        if self.offsets.ndim != 1 or len(self.offsets) < 1:
+            raise ValueError("offsets must hold n+1 >= 1 entries")
+        m = int(self.offsets[-1])
+        if m < 0 or len(self.cols) < m:
+            raise ValueError(f"cols holds {len(self.cols)} entries, offsets[-1] = {m}")
+        if self.w is not None and len(self.w) < m:
+            raise ValueError(f"w holds {len(self.w)} entries, offsets[-1] = {m}")
 
     @property
     def n(self) -> int:
@@ -168,12 +184,16 @@ class CSR:
         return int(self.offsets[-1]) if len(self.offsets) else 0
 
     def _c(self) -> _Csr:
+        w32 = self.w if self.w is not None and self.w.dtype == np.float32 else None
+        w64 = self.w if self.w is not None and self.w.dtype == np.float64 else None
         return _Csr(self.n, self.nnz, self.offsets.ctypes.data, self.cols.ctypes.data,
-                    self.w.ctypes.data if self.w is not None else None)
+                    w32.ctypes.data if w32 is not None else None,
+                    w64.ctypes.data if w64 is not None else None)
 
     @staticmethod
-    def from_edges(n: int, edges, weights=None) -> "CSR":
-        """Symmetric CSR from undirected edges (u, v[, w]); self-loops kept once."""
+    def from_edges(n: int, edges, weights=None, w64: bool = False) -> "CSR":
+        """Symmetric CSR from undirected edges (u, v[, w]); self-loops kept once.
+        Weights are stored as float32, or as float64 with w64=True."""
         e = np.asarray(edges, dtype=np.int64).reshape(-1, 2)
         u, v = e[:, 0], e[:, 1]
         ww = None if weights is None else np.asarray(weights, dtype=np.float64)
@@ -186,7 +206,8 @@ class CSR:
         off = np.zeros(n + 1, dtype=np.int64)
         np.add.at(off, ru + 1, 1)
         off = np.cumsum(off)
-        return CSR(off, rv.astype(np.int32), None if rw is None else rw[order].astype(np.float32))
+        return CSR(off, rv.astype(np.int32),
+                   None if rw is None else rw[order].astype(np.float64 if w64 else np.float32))
 
 
 def _take_host_csr(h: _HostCsr) -> CSR:
@@ -194,7 +215,9 @@ def _take_host_csr(h: _HostCsr) -> CSR:
     off = np.ctypeslib.as_array(h.offsets, shape=(n + 1,)).copy()
     cols = np.ctypeslib.as_array(h.cols, shape=(max(m, 1),))[:m].copy()
     w = None
-    if h.w:
+    if h.w64:
+        w = np.ctypeslib.as_array(h.w64, shape=(max(m, 1),))[:m].copy()
+    elif h.w:
         w = np.ctypeslib.as_array(h.w, shape=(max(m, 1),))[:m].copy()
     _lib.mgl_csr_free(C.byref(h))
     return CSR(off, cols, w)
@@ -223,8 +246,28 @@ def gen_sbm(n: int, n_comm: int, exponent=2.1, min_size=64, deg_in=24.0, deg_out
     return _take_host_csr(h)
 
 
+def gen_batch_graph(i: int) -> CSR:
+    """Graph i of the batched workload: vertex count log-uniform in [1,000,
+    200,000] (seed 4,000,000 + i); even i a planted partition (8-256
+    communities, power-law sizes, intra / inter mean degree 16 / 3), odd i an
+    R-MAT graph (edge factor uniform in [8, 32]); weights uniform (0.5, 1.5]."""
+    import math
+    rng = np.random.default_rng(4_000_000 + i)
+    n = int(round(math.exp(rng.uniform(math.log(1_000), math.log(200_000)))))
+    seed = 4_000_000 + i
+    if i % 2 == 0:
+        k = int(rng.integers(8, 257))
+        k = max(1, min(k, n // 8))
+        return gen_sbm(n, k, exponent=2.1, min_size=8, deg_in=16, deg_out=3, seed=seed,
+                       weights=1)
+    ef = float(rng.uniform(8, 32))
+    scale = max(1, math.ceil(math.log2(n)))
+    return gen_rmat(scale, ef, 0.57, 0.19, 0.19, seed=seed, weights=1, n_limit=n)
+
+
 def read_mtx(path: str) -> CSR:
-    """Matrix Market file with the reference loader's semantics."""
+    """Matrix Market file with the reference loader's semantics (general
+    banner: weights as float64, the reference's weight_t)."""
     h = _HostCsr()
     _chk(_lib.mgl_read_mtx(path.encode(), C.byref(h)))
     return _take_host_csr(h)
@@ -237,6 +280,8 @@ class Level:
     q: float
     iters: int
     qtraj: np.ndarray
+    q0: float = 0.0  # Q of the singleton start ("Loop 0")
+    iter_seconds: Optional[np.ndarray] = None  # device time of each iteration
     assign: Optional[np.ndarray] = None
     graph: Optional[CSR] = None  # input graph of this level (levels >= 1), weights as float64
     graph_w64: Optional[np.ndarray] = None
@@ -268,7 +313,10 @@ def _read_result(r, dump: Optional[str] = None, dump_binary: bool = False) -> Re
         qt = np.ctypeslib.as_array(qp, shape=(it,)).copy() if (it > 0 and qp) else np.zeros(0)
         ap = _lib.mgl_result_level_assign(r, l)
         assign = np.ctypeslib.as_array(ap, shape=(n,)).copy() if (ap and n > 0) else None
-        lev = Level(n=n, q=_lib.mgl_result_level_q(r, l), iters=it, qtraj=qt, assign=assign)
+        tp = _lib.mgl_result_level_iter_seconds(r, l)
+        ts = np.ctypeslib.as_array(tp, shape=(it,)).copy() if (it > 0 and tp) else np.zeros(0)
+        lev = Level(n=n, q=_lib.mgl_result_level_q(r, l), iters=it, qtraj=qt,
+                    q0=_lib.mgl_result_level_q0(r, l), iter_seconds=ts, assign=assign)
         cs = C.c_double()
         lev.seconds = _lib.mgl_result_level_seconds(r, l, C.byref(cs))
         lev.coarsen_seconds = cs.value
@@ -283,7 +331,7 @@ def _read_result(r, dump: Optional[str] = None, dump_binary: bool = False) -> Re
             lev.graph = CSR(off, cols, None)
             lev.graph_w64 = w64
         out.levels.append(lev)
-    n0 = out.levels[0].n if L else 0
+    n0 = _lib.mgl_result_vertices(r)
     fp = _lib.mgl_result_final_assign(r)
     out.final_assign = (np.ctypeslib.as_array(fp, shape=(n0,)).copy() if (fp and n0 > 0)
                         else np.zeros(0, np.int32))
@@ -367,6 +415,15 @@ class Context:
 
     def lds_bytes(self) -> int:
         return _lib.mgl_device_lds_bytes(self._h)
+
+    def modularity_dev(self, n: int, nnz: int, d_offsets: int, d_cols: int, d_w: int,
+                       d_assign: int) -> float:
+        """Modularity of a partition of a graph already in device memory
+        (device addresses as integers; d_w = 0 for unit weights)."""
+        q = C.c_double()
+        self._err(_lib.mgl_modularity_dev(self._h, n, nnz, d_offsets, d_cols, d_w or None,
+                                          d_assign, C.byref(q)))
+        return q.value
 
     def autotune(self, g: CSR, path: str):
         cg = g._c()
@@ -467,7 +524,7 @@ class HostGraph:
     def __init__(self, path_or_csr):
         self.csr = read_mtx(path_or_csr) if isinstance(path_or_csr, str) else path_or_csr
         w = self.csr.w
-        self.mass = float(self.csr.nnz if w is None else np.sum(w, dtype=np.float64))
+        self.mass = float(self.csr.nnz if w is None else np.sum(w.astype(np.float64)))
 
 
 class GpuGraph:

@@ -91,7 +91,7 @@ int main(int argc, char** argv) {
               : prec == "fp32"  ? MGL_PREC_FP32_HASH
                                 : MGL_PREC_AUTO;
   if (dump) o.keep_levels = o.keep_graphs = 1;
-  mgl_csr g = {h.n, h.nnz, h.offsets, h.cols, h.w};
+  mgl_csr g = {h.n, h.nnz, h.offsets, h.cols, h.w, h.w64};
   mgl_result* r = nullptr;
   st = mgl_louvain(ctx, &g, &o, &r);
   if (st != MGL_OK) {
@@ -109,15 +109,21 @@ int main(int argc, char** argv) {
     printf("|------------|------------|------------|------------|------------|\n");
     const int it = mgl_result_level_iters(r, l);
     const double* qt = mgl_result_level_qtraj(r, l);
+    const double* ts = mgl_result_level_iter_seconds(r, l);
     double cs = 0;
     const double ls = mgl_result_level_seconds(r, l, &cs);
-    const double per = it > 0 ? ls / it : 0;
-    double prev = (l == 0) ? NAN : mgl_result_level_q(r, l - 1);
+    // loop 0: the singleton start (louvain.cu:1536-1538)
+    double cur = mgl_result_level_q0(r, l);
+    printf("| %-10d | %-10f | %-10f | %-10f | %-10f |\n", 0, cur, 0., 0., 0.);
+    double loop_s = 0;
     for (int i = 0; i < it; ++i) {
-      const double dq = std::isnan(prev) ? 0.0 : qt[i] - prev;
-      printf("| %-10d | %-10f | %-10f | %-10f | %-10f |\n", i + 1, qt[i], dq, per, per * 1000);
-      prev = qt[i];
+      // dQ = new_Q - cur_Q, zero after a rejected proposal (louvain.cu:1642-1654)
+      const double t = ts ? ts[i] : (it > 0 ? ls / it : 0);
+      printf("| %-10d | %-10f | %-10f | %-10f | %-10f |\n", i + 1, qt[i], qt[i] - cur, t, t * 1000);
+      cur = qt[i];
+      loop_s += t;
     }
+    const double per = it > 0 ? loop_s / it : 0;
     printf("Main loop total execution time: %f s, %f ms\n", ls, ls * 1000);
     printf("The average execution time per loop: %f s, %f ms\n", per, per * 1000);
     if (l + 1 < L) printf("Coarsen graph execution time: %f s, %f ms\n", cs, cs * 1000);

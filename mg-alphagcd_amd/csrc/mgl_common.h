@@ -83,6 +83,8 @@ struct GState {
   double sa2p_d;
   int64_t moved;      // statistics
   int64_t distinct;
+  double q0;          // Q of the level's singleton start (reference "Loop 0" row)
+  int64_t tlast;      // device clock at the level start / the last decision
 };
 
 // ---------------------------------------------------------------------------

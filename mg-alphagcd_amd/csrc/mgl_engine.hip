@@ -41,6 +41,8 @@ struct Tuning {
   int hub_segmented = 1;      // 0: the hub path always scatters into buckets
   int hub_agg_threads = 512;  // threads per hub-aggregate workgroup (256, 512, 1024)
   int hub_agg_per_cu = 4;     // hub-aggregate workgroups sharing a CU's LDS (1..8)
+  bool table_check = true;    // false: LDS tables smaller than a bin's rows are accepted
+                              // (an overflow then fails the call; used by tests)
 };
 
 struct mgl_ctx {
@@ -48,6 +50,7 @@ struct mgl_ctx {
   hipStream_t st = nullptr;
   std::string err;
   int lds_max = 65536;
+  double clock_hz = 1.0e8;  // rate of the device clock read by wall_clock64()
   Tuning tune;  // settings of calls without a tuning file (the autotuner varies them)
 };
 
@@ -184,6 +187,8 @@ struct LevelOut {
   double seconds = 0;        // inner loop of the level (wall clock)
   double coarsen_seconds = 0;
   std::vector<double> qtraj;
+  double q0 = 0;                 // Q of the singleton start
+  std::vector<double> iter_secs; // device time of each iteration
   std::vector<int32_t> assign;
   int64_t gn = 0, gnnz = 0;
   std::vector<int64_t> goff;
@@ -227,7 +232,7 @@ struct Engine {
   bool rec32 = false;  // 8-byte records (unit mode, every graph's 2m < 2^31)
   vid* c = nullptr; vid* p = nullptr; vid* nx = nullptr;
   // per-graph
-  DBuf bgs, brun, btraj, bkeep, bdelta, bgather, bcounts;
+  DBuf bgs, brun, btraj, bttraj, bkeep, bdelta, bgather, bcounts;
   std::vector<GState> hgs;
   // bins of the current level
   std::vector<int64_t> bin_n, bin_off;
@@ -267,6 +272,18 @@ struct Engine {
   int64_t vcount() { return nlist >= 0 ? nlist : LG().n; }
   // device word: graphs still active after the last decision
   int* nactive() { return (int*)(bcounts.as<unsigned long long>() + 3 * MGL_MAX_BINS); }
+  // device word: set by a move kernel whose LDS table overflowed
+  int* err_word() { return (int*)(bcounts.as<unsigned long long>() + 4 * MGL_MAX_BINS + 3); }
+  mgl_status check_err() {
+    int e = 0;
+    HIPC(ctx, hipMemcpyAsync(&e, err_word(), 4, hipMemcpyDeviceToHost, st()));
+    HIPC(ctx, hipStreamSynchronize(st()));
+    if (e)
+      FAIL(ctx, MGL_DEVICE_ERROR,
+           "an LDS hash table of a move kernel overflowed (the bin layout's table is smaller "
+           "than a row's distinct communities)");
+    return MGL_OK;
+  }
   LevelGraph& LG() { return lg[cur]; }
 
   mgl_status alloc_state() {
@@ -293,9 +310,12 @@ struct Engine {
     HIPC(ctx, bdelta.ensure(8 * G));
     HIPC(ctx, bgather.ensure(8 * (G + 1) * 2));
     HIPC(ctx, btraj.ensure(sizeof(double) * (size_t)G * TRAJ_MAX));
+    HIPC(ctx, bttraj.ensure(sizeof(int64_t) * (size_t)G * TRAJ_MAX));
     // [0, MAX) bin counts, [MAX, 2 MAX) scatter cursors, [2 MAX, 3 MAX) distinct
-    // pairs per bin, [3 MAX] active graphs, [4 MAX, 4 MAX + 3) LDS probe counters
+    // pairs per bin, [3 MAX] active graphs, [4 MAX, 4 MAX + 3) LDS probe counters,
+    // [4 MAX + 3] LDS-table overflow flag
     HIPC(ctx, bcounts.ensure(8 * (MGL_MAX_BINS * 4 + 4)));
+    launch_fill_bytes0(bcounts.p, 8 * (MGL_MAX_BINS * 4 + 4), st());
     c = bc.as<vid>(); p = bp.as<vid>(); nx = bn.as<vid>();
     if (!ev_ok) {
       for (auto& x : ev) HIPC(ctx, hipEventCreate(&x));
@@ -531,6 +551,7 @@ struct Engine {
     A.stats = nullptr;
     A.probes = nullptr;
     A.pos0 = -1;
+    A.err = err_word();
     return A;
   }
 
@@ -642,7 +663,8 @@ struct Engine {
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 2], st()));
       launch_reduce(g.mode, vcount(), bvin.p, ba.p, gid, gs, st(), vlist());
       launch_fill_bytes0(nactive(), 4, st());
-      launch_decide(G, gs, o.threshold, o.max_iter, 1, ex, traj, TRAJ_MAX, 0, st(), nactive());
+      launch_decide(G, gs, o.threshold, o.max_iter, 1, ex, traj, TRAJ_MAX, 0, st(), nactive(),
+                    bttraj.as<int64_t>());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 3], st()));
       launch_fixup(vcount(), c, p, nx, gid, gs, st(), vlist());
       vid* t = c; c = p; p = nx; nx = t;
@@ -657,7 +679,8 @@ struct Engine {
       if (s) return s;
       launch_reduce(g.mode, vcount(), bvin.p, ba.p, gid, gs, st(), vlist());
       launch_fill_bytes0(nactive(), 4, st());
-      launch_decide(G, gs, o.threshold, o.max_iter, 0, ex, traj, TRAJ_MAX, 0, st(), nactive());
+      launch_decide(G, gs, o.threshold, o.max_iter, 0, ex, traj, TRAJ_MAX, 0, st(), nactive(),
+                    bttraj.as<int64_t>());
       if (timing) HIPC(ctx, hipEventRecord(ev[2 * MGL_MAX_BINS + 3], st()));
       launch_fixup(vcount(), c, p, nullptr, gid, gs, st(), vlist());
     }
@@ -965,7 +988,9 @@ static mgl_status copy_level_graph_to_host(mgl_ctx* ctx, LevelGraph& g, int64_t 
 // to coarsen, build the next level's union graph.
 mgl_status Engine::level_end(int level) {
   LevelGraph& g = LG();
-  mgl_status s = read_gs();
+  mgl_status s = check_err();
+  if (s) return s;
+  s = read_gs();
   if (s) return s;
   // trajectories: only the columns any graph used
   int tmax = 1;
@@ -974,6 +999,10 @@ mgl_status Engine::level_end(int level) {
   std::vector<double> traj((size_t)G * TRAJ_MAX);
   HIPC(ctx, hipMemcpy2DAsync(traj.data(), sizeof(double) * TRAJ_MAX, btraj.p,
                              sizeof(double) * TRAJ_MAX, sizeof(double) * tmax, G,
+                             hipMemcpyDeviceToHost, st()));
+  std::vector<int64_t> ttraj((size_t)G * TRAJ_MAX);
+  HIPC(ctx, hipMemcpy2DAsync(ttraj.data(), sizeof(int64_t) * TRAJ_MAX, bttraj.p,
+                             sizeof(int64_t) * TRAJ_MAX, sizeof(int64_t) * tmax, G,
                              hipMemcpyDeviceToHost, st()));
   std::vector<int32_t> hc;
   if (o.keep_levels && g.n > 0) {
@@ -990,6 +1019,10 @@ mgl_status Engine::level_end(int level) {
     lo.iters = hgs[i].iters;
     const int nt = std::min(lo.iters, TRAJ_MAX);
     lo.qtraj.assign(traj.begin() + (size_t)i * TRAJ_MAX, traj.begin() + (size_t)i * TRAJ_MAX + nt);
+    lo.q0 = hgs[i].q0;
+    lo.iter_secs.resize(nt);
+    for (int t = 0; t < nt; ++t)
+      lo.iter_secs[t] = (double)ttraj[(size_t)i * TRAJ_MAX + t] / ctx->clock_hz;
     if (o.keep_levels) {
       lo.assign.resize(lo.n);
       for (int64_t v = 0; v < lo.n; ++v) lo.assign[v] = hc[g.voff[i] + v] - (int32_t)g.voff[i];
@@ -1100,6 +1133,12 @@ mgl_status Engine::level_end(int level) {
   return MGL_OK;
 }
 
+__global__ void k_local_iota(int64_t n0, const int32_t* gid0, const int64_t* vbase, vid* fin) {
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n0;
+       v += (int64_t)gridDim.x * blockDim.x)
+    fin[v] = (vid)(v - vbase[gid0 ? gid0[v] : 0]);
+}
+
 mgl_status Engine::run(std::vector<mgl_result*>& out) {
   auto t0 = std::chrono::steady_clock::now();
   Q_host.assign(G, 0.0);
@@ -1107,6 +1146,16 @@ mgl_status Engine::run(std::vector<mgl_result*>& out) {
   phase.assign(G, 0);
   running.assign(G, 1);
   launch_iota(blab.as<vid>(), n0, st());
+  // final labels before any level (max_phases = 0): every vertex on its own
+  if (n0 > 0) {
+    DBuf dv;
+    HIPC(ctx, dv.ensure(8 * (G + 1)));
+    HIPC(ctx, hipMemcpyAsync(dv.p, vbase0.data(), 8 * (G + 1), hipMemcpyHostToDevice, st()));
+    const unsigned gr = (unsigned)std::min<int64_t>(8192, (n0 + 255) / 256 + 1);
+    k_local_iota<<<gr, 256, 0, st()>>>(n0, bgid0.as<int32_t>(), dv.as<int64_t>(),
+                                       bfin.as<vid>());
+    HIPC(ctx, hipStreamSynchronize(st()));
+  }
   int level = 0;
   while (true) {
     bool any = false;
@@ -1185,21 +1234,28 @@ static void parallel_for(int64_t n, F f) {
   for (auto& x : th) x.join();
 }
 
+// Input weights: doubles (w64, the reference's weight_t) when given, else
+// floats, else unit.
+static inline bool has_w(const mgl_csr& g) { return g.w64 != nullptr || g.w != nullptr; }
+static inline double wt(const mgl_csr& g, int64_t e) {
+  return g.w64 ? g.w64[e] : (g.w ? (double)g.w[e] : 1.0);
+}
+
 // Smallest S >= 0 such that every weight * 2^S is an integer, and the sum of
 // the weights in *total; -1 if a weight is not finite and positive.
-static int dyadic_scale(const float* w, int64_t nnz, long double* total) {
+static int dyadic_scale(const mgl_csr& g, long double* total) {
   int nt = 16;
   std::vector<int> sc(nt, 0);
   std::vector<int> bad(nt, 0);
   std::vector<long double> tot(nt, 0);
-  parallel_for(nnz, [&](int64_t b, int64_t e, int t) {
+  parallel_for(g.nnz, [&](int64_t b, int64_t e, int t) {
     int S = 0;
     long double s = 0;
     for (int64_t i = b; i < e; ++i) {
-      const float x = w[i];
-      if (!(x > 0.0f) || !std::isfinite(x)) { bad[t] = 1; return; }
+      const double x = wt(g, i);
+      if (!(x > 0.0) || !std::isfinite(x)) { bad[t] = 1; return; }
       int ex;
-      const double m = std::frexp((double)x, &ex);
+      const double m = std::frexp(x, &ex);
       const uint64_t mi = (uint64_t)std::ldexp(m, 53);
       const int need = 53 - __builtin_ctzll(mi) - ex;
       if (need > S) S = need;
@@ -1268,7 +1324,10 @@ static mgl_status validate_csr(mgl_ctx* ctx, const mgl_csr* g, int level) {
     for (int64_t i = b; i < e; ++i) {
       const int32_t x = g->cols[i];
       if (x < 0 || x >= n) { bad[t] = 2; return; }
-      if (g->w && !(g->w[i] >= 0.0f && std::isfinite(g->w[i]))) { bad[t] = 3; return; }
+      if (has_w(*g)) {
+        const double x = wt(*g, i);
+        if (!(x >= 0.0 && std::isfinite(x))) { bad[t] = 3; return; }
+      }
     }
   });
   for (int x : bad) {
@@ -1284,7 +1343,7 @@ static mgl_status validate_csr(mgl_ctx* ctx, const mgl_csr* g, int level) {
           const int32_t v = g->cols[i];
           bool found = false;
           for (int64_t j = g->offsets[v]; j < g->offsets[v + 1]; ++j)
-            if (g->cols[j] == (int32_t)u && (!g->w || g->w[j] == g->w[i])) { found = true; break; }
+            if (g->cols[j] == (int32_t)u && wt(*g, j) == wt(*g, i)) { found = true; break; }
           if (!found) { asym[t] = 1; break; }
         }
     });
@@ -1302,7 +1361,7 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
     if (s) return s;
   }
   bool weighted = false;
-  for (int i = 0; i < G; ++i) weighted |= graphs[i].w != nullptr;
+  for (int i = 0; i < G; ++i) weighted |= has_w(graphs[i]);
   prec = o.precision;
   int S = 0;
   if (weighted && (prec == MGL_PREC_AUTO || prec == MGL_PREC_EXACT)) {
@@ -1310,11 +1369,11 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
     // count 2^S each) must stay below 2^62
     std::vector<long double> tot(G, 0.0L);
     for (int i = 0; i < G && S >= 0; ++i) {
-      if (!graphs[i].w) {
+      if (!has_w(graphs[i])) {
         tot[i] = (long double)graphs[i].nnz;
         continue;
       }
-      const int si = dyadic_scale(graphs[i].w, graphs[i].nnz, &tot[i]);
+      const int si = dyadic_scale(graphs[i], &tot[i]);
       S = si < 0 ? -1 : std::max(S, si);
     }
     for (int i = 0; i < G && S >= 0; ++i)
@@ -1351,24 +1410,26 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
     const mgl_csr& gr = graphs[i];
     GState& s = hgs[i];
     if (prec == MGL_PREC_EXACT) {
-      if (!gr.w) {
+      if (!has_w(gr)) {
         s.mass_i = gr.nnz * ((int64_t)1 << S);  // unit weights carry 2^S in a weighted batch
       } else {
         std::vector<int64_t> part(16, 0);
         const int sc = S;
         parallel_for(gr.nnz, [&](int64_t b, int64_t e, int t) {
           int64_t acc = 0;
-          for (int64_t j = b; j < e; ++j) acc += (int64_t)std::ldexp((double)gr.w[j], sc);
+          for (int64_t j = b; j < e; ++j) acc += (int64_t)std::ldexp(wt(gr, j), sc);
           part[t] = acc;
         });
         for (auto x : part) s.mass_i += x;
       }
       s.mass = (double)s.mass_i;
     } else {
+      // FP64 sum in entry order (the reference's thrust::reduce over the
+      // weights, host_graph.cpp:112-114)
       double m = 0;
-      if (!gr.w) m = (double)gr.nnz;
+      if (!has_w(gr)) m = (double)gr.nnz;
       else
-        for (int64_t j = 0; j < gr.nnz; ++j) m += (double)gr.w[j];
+        for (int64_t j = 0; j < gr.nnz; ++j) m += wt(gr, j);
       s.mass = m;
     }
   }
@@ -1403,14 +1464,23 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
   HIPC(ctx, hipMemcpyAsync(g.off.as<int64_t>() + n0, &nnz0, 8, hipMemcpyHostToDevice, st()));
   if (mode0 != MODE_UNIT) {
     HIPC(ctx, g.w.ensure(wsize(mode0) * std::max<int64_t>(nnz0, 1)));
+    // input weights are staged one graph at a time (stream order keeps a
+    // graph's cast ahead of the next graph's copy into the same buffer)
     DBuf stage;
-    if (weighted) HIPC(ctx, stage.ensure(4 * std::max<int64_t>(nnz0, 1)));
+    int64_t smax = 1;
+    for (int i = 0; i < G; ++i)
+      if (has_w(graphs[i])) smax = std::max<int64_t>(smax, graphs[i].nnz);
+    if (weighted) HIPC(ctx, stage.ensure(8 * smax));
     for (int i = 0; i < G; ++i) {
       const mgl_csr& gr = graphs[i];
       if (gr.nnz == 0) continue;
-      if (gr.w) {
-        HIPC(ctx, hipMemcpyAsync(stage.as<float>() + eb[i], gr.w, 4 * gr.nnz,
-                                 hipMemcpyHostToDevice, st()));
+      if (gr.w64) {
+        HIPC(ctx, hipMemcpyAsync(stage.p, gr.w64, 8 * gr.nnz, hipMemcpyHostToDevice, st()));
+        launch_cast_weights_f64(mode0, stage.as<double>(), (char*)g.w.p + wsize(mode0) * eb[i],
+                                gr.nnz, S, st());
+        continue;
+      } else if (gr.w) {
+        HIPC(ctx, hipMemcpyAsync(stage.p, gr.w, 4 * gr.nnz, hipMemcpyHostToDevice, st()));
       } else if (mode0 == MODE_INT) {
         // unit weights of an unweighted graph inside a weighted batch
         std::vector<int64_t> one(gr.nnz, (int64_t)1 << S);
@@ -1422,7 +1492,7 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
         k_fill_one<<<grid_n(gr.nnz), 256, 0, st()>>>(mode0, (char*)g.w.p + wsize(mode0) * eb[i], gr.nnz);
         continue;
       }
-      launch_cast_weights(mode0, stage.as<float>() + eb[i], (char*)g.w.p + wsize(mode0) * eb[i],
+      launch_cast_weights(mode0, stage.as<float>(), (char*)g.w.p + wsize(mode0) * eb[i],
                           gr.nnz, S, st());
     }
     HIPC(ctx, hipStreamSynchronize(st()));
@@ -1491,6 +1561,10 @@ mgl_status mgl_create(mgl_ctx** out, int device) {
           hipSuccess &&
       lds > 0)
     ctx->lds_max = lds;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
+      khz > 0)
+    ctx->clock_hz = 1000.0 * khz;
   (void)hipGetLastError();
   move_kernels_allow_lds(ctx->lds_max);
   hub_allow_lds(ctx->lds_max);
@@ -1507,6 +1581,14 @@ void mgl_destroy(mgl_ctx* ctx) {
 }
 
 const char* mgl_last_error(const mgl_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+}  // extern "C"
+
+// context access for the other host units of the library
+void mgl_set_error(mgl_ctx* ctx, const std::string& msg) { ctx->err = msg; }
+int mgl_ctx_device(const mgl_ctx* ctx) { return ctx->device; }
+hipStream_t mgl_ctx_stream(const mgl_ctx* ctx) { return ctx->st; }
+
+extern "C" {
 
 static mgl_status check_opts(mgl_ctx* ctx, const mgl_opts* o) {
   if (o->version < 0 || o->version > 2) FAIL(ctx, MGL_INVALID_ARG, "version must be 0, 1 or 2");
@@ -1572,6 +1654,13 @@ int mgl_result_level_iters(const mgl_result* r, int l) {
 const double* mgl_result_level_qtraj(const mgl_result* r, int l) {
   return (r && l >= 0 && l < (int)r->levels.size()) ? r->levels[l].qtraj.data() : nullptr;
 }
+double mgl_result_level_q0(const mgl_result* r, int l) {
+  return (r && l >= 0 && l < (int)r->levels.size()) ? r->levels[l].q0 : NAN;
+}
+const double* mgl_result_level_iter_seconds(const mgl_result* r, int l) {
+  if (!r || l < 0 || l >= (int)r->levels.size() || r->levels[l].iter_secs.empty()) return nullptr;
+  return r->levels[l].iter_secs.data();
+}
 const int32_t* mgl_result_level_assign(const mgl_result* r, int l) {
   if (!r || l < 0 || l >= (int)r->levels.size() || r->levels[l].assign.size() != (size_t)r->levels[l].n)
     return nullptr;
@@ -1588,6 +1677,9 @@ mgl_status mgl_result_level_graph(const mgl_result* r, int l, int64_t* n, int64_
   if (col) *col = lo.gcol.data();
   if (w) *w = lo.gw.data();
   return MGL_OK;
+}
+int64_t mgl_result_vertices(const mgl_result* r) {
+  return r ? (int64_t)r->final_assign.size() : 0;
 }
 const int32_t* mgl_result_final_assign(const mgl_result* r) {
   return r ? r->final_assign.data() : nullptr;
@@ -1656,12 +1748,47 @@ struct mgl_session {
   Engine* E = nullptr;
   int it = 0;
   bool coarsen_ready = false;
+  bool restart = false;  // the last pass rejected: the next pass restarts the level's loop
   mgl_pass_stats st{};
-  std::vector<int64_t> hoff;  // host copy of the level-0 offsets (statistics)
+  int64_t bin_e[MGL_MAX_BINS] = {};  // CSR entries of each bin's vertices (statistics)
   // counters of the move of pass count_it, from mgl_session_count
   int count_it = -1;
   unsigned long long cnt[MGL_MAX_BINS + 3] = {};
 };
+
+// CSR entries of the vertices perm[0 .. nb): one atomic per wavefront.
+__global__ __launch_bounds__(256) void k_bin_entries(const eid* __restrict__ off,
+                                                     const vid* __restrict__ perm, int64_t nb,
+                                                     unsigned long long* out) {
+  int64_t acc = 0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nb; i += (int64_t)gridDim.x * 256) {
+    const vid v = perm[i];
+    acc += off[v + 1] - off[v];
+  }
+  acc = group_sum<64>(acc);
+  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(out, (unsigned long long)acc);
+}
+
+// Per-bin entry counts of the session's level (statistics only; computed when
+// the bins are made, outside any timed pass).
+static mgl_status session_bin_entries(mgl_session* s) {
+  Engine& E = *s->E;
+  mgl_ctx* ctx = s->ctx;
+  DBuf d;
+  HIPC(ctx, d.ensure(8 * MGL_MAX_BINS));
+  launch_fill_bytes0(d.p, 8 * MGL_MAX_BINS, ctx->st);
+  for (int b = 0; b < E.L.nb; ++b)
+    if (E.bin_n[b] > 0)
+      k_bin_entries<<<grid_n(E.bin_n[b]), 256, 0, ctx->st>>>(
+          E.LG().off.as<eid>(), E.bperm.as<vid>() + E.bin_off[b], E.bin_n[b],
+          d.as<unsigned long long>() + b);
+  HIPC(ctx, hipGetLastError());
+  unsigned long long h[MGL_MAX_BINS];
+  HIPC(ctx, hipMemcpyAsync(h, d.p, 8 * MGL_MAX_BINS, hipMemcpyDeviceToHost, ctx->st));
+  HIPC(ctx, hipStreamSynchronize(ctx->st));
+  for (int b = 0; b < MGL_MAX_BINS; ++b) s->bin_e[b] = b < E.L.nb ? (int64_t)h[b] : 0;
+  return MGL_OK;
+}
 
 __global__ void k_reactivate(GState* gs) {
   GState& s = gs[0];
@@ -1719,6 +1846,7 @@ mgl_status mgl_session_create(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* op
   ss->E->Q_old.assign(1, -1.0);
   ss->E->phase.assign(1, 0);
   s = ss->E->level_start();
+  if (!s) s = session_bin_entries(ss);
   if (s) { delete ss->E; delete ss; return s; }
   HIPC(ctx, hipStreamSynchronize(ctx->st));
   *out = ss;
@@ -1739,7 +1867,10 @@ mgl_status mgl_session_reset(mgl_session* s, double* q0) {
   Engine& E = *s->E;
   mgl_status r = E.level_start();
   if (r) return r;
+  r = session_bin_entries(s);
+  if (r) return r;
   s->it = 0;
+  s->restart = false;
   r = E.read_gs();
   if (r) return r;
   if (q0) *q0 = E.hgs[0].q_cur;
@@ -1756,12 +1887,25 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   mgl_ctx* ctx = s->ctx;
   HIPC(ctx, hipSetDevice(ctx->device));
   Engine& E = *s->E;
+  mgl_status r;
+  if (s->restart) {
+    // the previous pass rejected its proposal, which ends a level: the loop
+    // restarts from the committed assignment with rebuilt community totals
+    k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
+    r = E.recompute_a(E.c);
+    if (r) return r;
+    r = session_prime(s);
+    if (r) return r;
+    s->restart = false;
+  }
   unsigned long long* dist = E.bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS;
   k_zero_stats<<<1, MGL_MAX_BINS, 0, ctx->st>>>(E.bgs.as<GState>(), dist);
   E.timing = true;
   s->it += 1;
-  mgl_status r = E.iteration(s->it);
+  r = E.iteration(s->it);
   E.timing = false;
+  if (r) return r;
+  r = E.check_err();
   if (r) return r;
   r = E.read_gs();
   if (r) return r;
@@ -1787,21 +1931,6 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   HIPC(ctx, hipEventElapsedTime(&ms, ev[B + 0], ev[B + 4]));
   st.pass_ms = ms;
   st.dominant_ms = -1;
-  // CSR entries of a bin's vertices
-  auto bin_entries = [&](int b, int64_t& ents) -> mgl_status {
-    std::vector<int32_t> perm(E.bin_n[b]);
-    HIPC(ctx, hipMemcpy(perm.data(), E.bperm.as<vid>() + E.bin_off[b], 4 * E.bin_n[b],
-                        hipMemcpyDeviceToHost));
-    std::vector<int64_t>& hoff = s->hoff;
-    const LevelGraph& g = E.LG();
-    if ((int64_t)hoff.size() != g.n + 1) {
-      hoff.resize(g.n + 1);
-      HIPC(ctx, hipMemcpy(hoff.data(), g.off.p, 8 * (g.n + 1), hipMemcpyDeviceToHost));
-    }
-    ents = 0;
-    for (int32_t v : perm) ents += hoff[v + 1] - hoff[v];
-    return MGL_OK;
-  };
   // counters of this pass's move, if mgl_session_count ran on the same state
   unsigned long long dh[MGL_MAX_BINS + 3] = {};
   if (s->count_it == s->it) memcpy(dh, s->cnt, sizeof(dh));
@@ -1825,16 +1954,14 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   if (st.dominant_ms >= 0) {
     const int b = st.dominant_bin;
     st.dominant_vertices = E.bin_n[b];
-    r = bin_entries(b, st.dominant_entries);
-    if (r) return r;
+    st.dominant_entries = s->bin_e[b];
     st.dominant_distinct = (int64_t)dh[b];
     move_kernel_name(E.LG().mode, E.rec32_now() ? 1 : 0, E.L.b[b], st.dominant_kernel,
                      (int)sizeof(st.dominant_kernel));
   }
   if (hub_bin >= 0) {
     st.hub_vertices = E.bin_n[hub_bin];
-    r = bin_entries(hub_bin, st.hub_entries);
-    if (r) return r;
+    st.hub_entries = s->bin_e[hub_bin];
     st.hub_distinct = (int64_t)dh[hub_bin];
   }
   st.distinct = 0;
@@ -1844,16 +1971,9 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   st.lds_multi_probe = (int64_t)dh[MGL_MAX_BINS + 2];
   st.moved = gs.moved;
   st.q = gs.q_cur;
-  // a rejected proposal ends the level; a session keeps passing, so it
-  // re-activates with community totals rebuilt from the committed assignment
-  if (!gs.active) {
-    k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
-    r = E.recompute_a(E.c);
-    if (r) return r;
-    r = session_prime(s);
-    if (r) return r;
-    HIPC(ctx, hipStreamSynchronize(ctx->st));
-  }
+  // a rejected proposal ends the level; the next pass restarts the loop (a
+  // caller that times passes can call mgl_session_reset in between instead)
+  s->restart = !gs.active;
   return MGL_OK;
 }
 
@@ -1926,6 +2046,8 @@ mgl_status mgl_session_move(mgl_session* s, int up, int32_t* c_new) {
   k_reactivate<<<1, 1, 0, ctx->st>>>(E.bgs.as<GState>());
   launch_copy_vid(E.c, E.p, n, ctx->st);  // isolated vertices keep their community
   mgl_status r = E.move(E.c, E.p, up ? 1 : 0, nullptr, false);
+  if (r) return r;
+  r = E.check_err();
   if (r) return r;
   HIPC(ctx, hipMemcpyAsync(c_new, E.p, 4 * n, hipMemcpyDeviceToHost, ctx->st));
   HIPC(ctx, hipStreamSynchronize(ctx->st));
@@ -2067,8 +2189,11 @@ mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64
 // Tuning file (text, written by mgl_autotune): lines starting with '#' and
 // empty lines are skipped; "hub_bucket N", "hub_bucket_unclamped N",
 // "hub_segmented 0|1", "hub_agg_threads 256|512|1024" and "hub_agg_per_cu 1..8"
-// set the hub path; every other line is one bin, "upper shape s size", in
-// increasing order of upper (-1: unbounded). Anything else is an error.
+// set the hub path; "table_check 0" accepts tables smaller than their bin's rows
+// (a move that overflows one then fails with MGL_DEVICE_ERROR); every other line
+// is one bin, "upper shape s size", in increasing order of upper (-1:
+// unbounded), whose shape, lane or thread count and table size must name a
+// compiled kernel. Anything else is an error.
 static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
   FILE* f = fopen(path, "r");
   if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot open tuning file %s", path);
@@ -2094,6 +2219,8 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
       tu.hub_bucket_clamp = false;
     } else if (sscanf(p, "hub_segmented %d %c", &hs, &extra) == 1) {
       tu.hub_segmented = hs != 0;
+    } else if (sscanf(p, "table_check %d %c", &hs, &extra) == 1) {
+      tu.table_check = hs != 0;
     } else if (sscanf(p, "hub_agg_threads %d %c", &hs, &extra) == 1) {
       if (hs != 256 && hs != 512 && hs != 1024) bad = "hub_agg_threads";
       tu.hub_agg_threads = hs;
@@ -2116,16 +2243,26 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
     const BinSpec& b = L.b[i];
     if (b.upper <= L.b[i - 1].upper) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bounds", path);
     if (b.shape == SHAPE_GROUP) {
+      if (b.s != 4 && b.s != 8 && b.s != 16 && b.s != 32 && b.s != 64)
+        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d lanes per vertex %d", path, i, b.s);
       // a table of 2^size slots must hold every distinct community of a row
-      if (b.size < 1 || b.size > 15 || b.upper > ((int64_t)1 << b.size))
+      if (b.size < 1 || b.size > 15 || (tu.table_check && b.upper > ((int64_t)1 << b.size)))
         FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table too small", path, i);
     } else if (b.shape == SHAPE_BLOCK) {
-      if (b.size < 1 || b.upper > b.size || (b.s != 256 && b.s != 512 && b.s != 1024))
+      if (b.s != 256 && b.s != 512 && b.s != 1024)
+        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d threads %d", path, i, b.s);
+      if (b.size < 1 || (tu.table_check && b.upper > b.size))
         FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table too small", path, i);
     } else if (b.shape == SHAPE_LANE) {
       if (b.upper > 16) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: lane bin above 16", path);
     } else if (b.shape != SHAPE_GLOBAL) {
       FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d shape", path, i);
+    }
+    if (b.shape == SHAPE_GROUP || b.shape == SHAPE_BLOCK || b.shape == SHAPE_LANE) {
+      // every mode instantiates the same shapes
+      char kn[128];
+      if (b.upper == INT64_MAX || move_kernel_name(MODE_UNIT, 1, b, kn, (int)sizeof(kn)) != 0)
+        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d has no compiled kernel", path, i);
     }
   }
   if (L.b[L.nb - 1].shape != SHAPE_GLOBAL)

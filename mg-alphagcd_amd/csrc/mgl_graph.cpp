@@ -147,7 +147,11 @@ mgl_status build_csr(std::vector<uint64_t>& keys, int64_t n, int weights, uint64
   out->offsets = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
   out->cols = (int32_t*)malloc(sizeof(int32_t) * (m > 0 ? m : 1));
   out->w = weights ? (float*)malloc(sizeof(float) * (m > 0 ? m : 1)) : nullptr;
-  if (!out->offsets || !out->cols || (weights && !out->w)) return MGL_OOM;
+  out->w64 = nullptr;
+  if (!out->offsets || !out->cols || (weights && !out->w)) {
+    mgl_csr_free(out);
+    return MGL_OOM;
+  }
   // offsets: the first entry of each row, found by binary search per row
   pfor(n + 1, [&](int64_t lo, int64_t hi, int) {
     for (int64_t v = lo; v < hi; ++v)
@@ -172,9 +176,11 @@ void mgl_csr_free(mgl_host_csr* g) {
   free(g->offsets);
   free(g->cols);
   free(g->w);
+  free(g->w64);
   g->offsets = nullptr;
   g->cols = nullptr;
   g->w = nullptr;
+  g->w64 = nullptr;
   g->n = g->nnz = 0;
 }
 
@@ -404,8 +410,12 @@ mgl_status mgl_read_mtx(const char* path, mgl_host_csr* out) {
   out->nnz = m;
   out->offsets = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
   out->cols = (int32_t*)malloc(sizeof(int32_t) * (m > 0 ? m : 1));
-  out->w = symmetric ? nullptr : (float*)malloc(sizeof(float) * (m > 0 ? m : 1));
-  if (!out->offsets || !out->cols || (!symmetric && !out->w)) return MGL_OOM;
+  // weights stay doubles, as the reference parses them into weight_t (:243-250)
+  out->w64 = symmetric ? nullptr : (double*)malloc(sizeof(double) * (m > 0 ? m : 1));
+  if (!out->offsets || !out->cols || (!symmetric && !out->w64)) {
+    mgl_csr_free(out);
+    return MGL_OOM;
+  }
   // rows sorted by column (stable for equal (row, col), which keeps duplicates)
   std::vector<int64_t> idx(m);
   for (int64_t i = 0; i < m; ++i) idx[i] = i;
@@ -414,7 +424,7 @@ mgl_status mgl_read_mtx(const char* path, mgl_host_csr* out) {
     const uint64_t k = keys[idx[i]];
     out->offsets[(k >> 32) + 1]++;
     out->cols[i] = (int32_t)(uint32_t)k;
-    if (!symmetric) out->w[i] = (float)wv[idx[i]];
+    if (!symmetric) out->w64[i] = wv[idx[i]];
   }
   for (int64_t v = 0; v < n; ++v) out->offsets[v + 1] += out->offsets[v];
   return MGL_OK;

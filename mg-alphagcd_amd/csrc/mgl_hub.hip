@@ -525,6 +525,7 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
   }
   // workgroup argmax (cnt is uniform, every thread reaches this point)
   if constexpr (CNT) flush_probes(A.probes, pc);
+  flag_overflow(A.err, pc);
   group_argmax<64>(best, dst, small);
   occ = group_sum<64>(occ);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;

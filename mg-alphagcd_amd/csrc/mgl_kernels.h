@@ -50,6 +50,7 @@ struct MoveArgs {
   unsigned long long* probes; // [3] LDS hash inserts, probes, multi-probe inserts (nullable)
   int64_t pos0;       // >= 0: perm[i] has record position pos0 + i (records in degree-bin
                       // order, bins not compacted); < 0: look the position up in rpos
+  int* err;           // set to 1 when an LDS table insert finds no free slot (nullable)
 };
 
 // Launch the move kernel of one GROUP or BLOCK bin. perm points at the bin's
@@ -178,9 +179,11 @@ void launch_reduce(int mode, int64_t n, const void* vin, const void* a, const in
 // singleton reduction, q_cur = Q(singletons).
 void launch_level_begin(int ng, GState* gs, const int32_t* run, hipStream_t st);
 void launch_level_init(int ng, GState* gs, int exact, hipStream_t st);
-// nactive (optional) receives the number of graphs still active afterwards.
+// nactive (optional) receives the number of graphs still active afterwards;
+// ttraj (optional, [ng * traj_max]) the device clock ticks of each iteration.
 void launch_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
-                   double* traj, int traj_max, int pre, hipStream_t st, int* nactive = nullptr);
+                   double* traj, int traj_max, int pre, hipStream_t st, int* nactive = nullptr,
+                   int64_t* ttraj = nullptr);
 void launch_fixup(int64_t n, vid* c, vid* p, vid* nx, const int32_t* gid, const GState* gs,
                   hipStream_t st, const vid* list = nullptr);
 // active-set compaction of a batch
@@ -194,6 +197,8 @@ void launch_copy_vid(const vid* src, vid* dst, int64_t n, hipStream_t st);
 void launch_fill_bytes0(void* p, size_t bytes, hipStream_t st);
 void launch_cast_weights(int to_mode, const float* w, void* out, int64_t nnz, int scale,
                          hipStream_t st);
+void launch_cast_weights_f64(int to_mode, const double* w, void* out, int64_t nnz, int scale,
+                             hipStream_t st);
 void launch_gid_fill(const int64_t* voff, int ng, int32_t* gid, hipStream_t st);
 
 // ---- coarsening (mgl_coarsen.hip) ------------------------------------------

@@ -755,7 +755,11 @@ __global__ void k_level_init(int ng, GState* gs, int exact) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   GState& s = gs[g];
-  if (s.active) s.q_cur = q_of(s, exact != 0, false);
+  if (s.active) {
+    s.q_cur = q_of(s, exact != 0, false);
+    s.q0 = s.q_cur;
+    s.tlast = wall_clock64();
+  }
   reset_acc(s);
 }
 void launch_level_init(int ng, GState* gs, int exact, hipStream_t st) {
@@ -765,18 +769,20 @@ void launch_level_init(int ng, GState* gs, int exact, hipStream_t st) {
 // pre = 1: after the first move of a level (fused mode) only stores sum a^2 of
 // the first proposal; no decision is taken.
 __device__ void decide_one(GState& s, int g, double thr, int max_iter, int fused, int exact,
-                           double* traj, int traj_max, int pre);
+                           double* traj, int64_t* ttraj, int traj_max, int pre);
 __global__ void k_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
-                         double* traj, int traj_max, int pre, int* nactive) {
+                         double* traj, int64_t* ttraj, int traj_max, int pre, int* nactive) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= ng) return;
   GState& s = gs[g];
   if (!s.active) { s.fix = 0; reset_acc(s); return; }
-  decide_one(s, g, thr, max_iter, fused, exact, traj, traj_max, pre);
+  decide_one(s, g, thr, max_iter, fused, exact, traj, ttraj, traj_max, pre);
   if (nactive && s.active) atomicAdd(nactive, 1);
 }
+// Besides the decision, the device clock ticks since the level start or the
+// previous decision go to ttraj (the iteration's time).
 __device__ void decide_one(GState& s, int g, double thr, int max_iter, int fused, int exact,
-                           double* traj, int traj_max, int pre) {
+                           double* traj, int64_t* ttraj, int traj_max, int pre) {
   if (pre) {
     s.sa2p_lo = s.sa2_lo; s.sa2p_hi = s.sa2_hi; s.sa2p_d = s.sa2_d;
     s.fix = 0;
@@ -785,6 +791,9 @@ __device__ void decide_one(GState& s, int g, double thr, int max_iter, int fused
   }
   const double Q = q_of(s, exact != 0, fused != 0);
   const int it = s.iters++;
+  const int64_t now = wall_clock64();
+  if (ttraj && it < traj_max) ttraj[(int64_t)g * traj_max + it] = now - s.tlast;
+  s.tlast = now;
   if ((Q - s.q_cur) > thr) {
     s.q_cur = Q;
     if (traj && it < traj_max) traj[(int64_t)g * traj_max + it] = Q;
@@ -802,9 +811,10 @@ __device__ void decide_one(GState& s, int g, double thr, int max_iter, int fused
 }
 
 void launch_decide(int ng, GState* gs, double thr, int max_iter, int fused, int exact,
-                   double* traj, int traj_max, int pre, hipStream_t st, int* nactive) {
-  k_decide<<<(ng + 255) / 256, 256, 0, st>>>(ng, gs, thr, max_iter, fused, exact, traj, traj_max,
-                                             pre, nactive);
+                   double* traj, int traj_max, int pre, hipStream_t st, int* nactive,
+                   int64_t* ttraj) {
+  k_decide<<<(ng + 255) / 256, 256, 0, st>>>(ng, gs, thr, max_iter, fused, exact, traj, ttraj,
+                                             traj_max, pre, nactive);
 }
 
 // Buffer fix-up after a decision:
@@ -925,23 +935,33 @@ void launch_fill_bytes0(void* p, size_t bytes, hipStream_t st) {
   if (nw > 0) k_zero<<<grid_for(nw, 256, 8192), 256, 0, st>>>((uint32_t*)p, nw);
 }
 
-// float input weights -> level-0 storage of the chosen mode. MODE_INT scales
-// by 2^scale (exact for dyadic weights, checked on the host).
-__global__ void k_cast(int to_mode, const float* w, void* out, int64_t nnz, double mul) {
+// Input weights (float or double) -> level-0 storage of the chosen mode.
+// MODE_INT scales by 2^scale (exact for dyadic weights, checked on the host);
+// MODE_F32 rounds a double to the nearest float.
+template <class T>
+__global__ void k_cast(int to_mode, const T* w, void* out, int64_t nnz, double mul) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nnz;
        i += (int64_t)gridDim.x * blockDim.x) {
     const double x = (double)w[i];
     if (to_mode == MODE_INT) ((int64_t*)out)[i] = (int64_t)(x * mul);
     else if (to_mode == MODE_F64) ((double*)out)[i] = x;
-    else ((float*)out)[i] = w[i];
+    else ((float*)out)[i] = (float)w[i];
   }
+}
+static double pow2(int scale) {
+  double mul = 1.0;
+  for (int i = 0; i < scale; ++i) mul *= 2.0;
+  return mul;
 }
 void launch_cast_weights(int to_mode, const float* w, void* out, int64_t nnz, int scale,
                          hipStream_t st) {
   if (nnz <= 0) return;
-  double mul = 1.0;
-  for (int i = 0; i < scale; ++i) mul *= 2.0;
-  k_cast<<<grid_for(nnz, 256, 8192), 256, 0, st>>>(to_mode, w, out, nnz, mul);
+  k_cast<float><<<grid_for(nnz, 256, 8192), 256, 0, st>>>(to_mode, w, out, nnz, pow2(scale));
+}
+void launch_cast_weights_f64(int to_mode, const double* w, void* out, int64_t nnz, int scale,
+                             hipStream_t st) {
+  if (nnz <= 0) return;
+  k_cast<double><<<grid_for(nnz, 256, 8192), 256, 0, st>>>(to_mode, w, out, nnz, pow2(scale));
 }
 
 __global__ void k_gid_fill(const int64_t* voff, int ng, int32_t* gid) {

@@ -55,7 +55,7 @@ __device__ __forceinline__ void ins_tab(vid* K, ST* V, AT* Av, vid key, ST w, AT
     }
     h = probe_next(h, st, H);
   }
-  pc.add(H);
+  pc.over(H);
 }
 template <class ST, class AT>
 __device__ __forceinline__ void ins_pow2(vid* K, ST* V, AT* Av, vid key, ST w, AT ak, int lg,
@@ -178,6 +178,7 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
     }
   }
   if constexpr (CNT) flush_probes(A.probes, pc);
+  flag_overflow(A.err, pc);
   if (CNT && A.stats) {
     __syncthreads();
     if (threadIdx.x == 0 && sdist) atomicAdd(A.stats, sdist);
@@ -219,6 +220,7 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   scan_row<M, AT, false>(A, vp, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, K, V, Av, H, eici,
                          selfw, pc);
   if constexpr (CNT) flush_probes(A.probes, pc);
+  flag_overflow(A.err, pc);
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
   selfw = block_sum<BS>(selfw, rs);
@@ -346,6 +348,7 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
     }
   }
   if constexpr (CNT) flush_probes(A.probes, pc);
+  flag_overflow(A.err, pc);
   if (CNT && A.stats) {
     __syncthreads();
     if (threadIdx.x == 0 && sdist) atomicAdd(A.stats, sdist);
@@ -383,6 +386,7 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   own_row<M, AT, IT, TB>(A, vp, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, tb, H, false, 0, own,
                          oa, eici, selfw, pc);
   if constexpr (CNT) flush_probes(A.probes, pc);
+  flag_overflow(A.err, pc);
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
   selfw = block_sum<BS>(selfw, rs);

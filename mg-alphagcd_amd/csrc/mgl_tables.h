@@ -26,14 +26,27 @@ __host__ __device__ __forceinline__ double gain(double e, double eici, double ki
 // Per-thread statistics of LDS hash inserts: inserts, compare-and-swap probes
 // and inserts that needed more than one probe. Summed per wavefront and added
 // to A.probes[0..2] when statistics are requested.
+// `fail` records an insert that found every slot taken by other keys (a table
+// smaller than the row's distinct communities); flag_overflow reports it.
 struct ProbeCount {
-  uint32_t ins = 0, probes = 0, multi = 0;
+  uint32_t ins = 0, probes = 0, multi = 0, fail = 0;
   __device__ __forceinline__ void add(uint32_t np) {
     ++ins;
     probes += np;
     multi += np > 1u ? 1u : 0u;
   }
+  __device__ __forceinline__ void over(uint32_t np) {
+    add(np);
+    fail = 1;
+  }
 };
+// A dropped entry makes the row's result wrong: the launch sets *err and the
+// host turns it into MGL_DEVICE_ERROR. The built-in layouts and every tuning
+// file that passes validation size the tables for their bins' rows, so this
+// does not happen unless that validation is switched off.
+__device__ __forceinline__ void flag_overflow(int* err, const ProbeCount& pc) {
+  if (pc.fail && err) atomicOr(err, 1);
+}
 // Every lane of the wavefront must call this (shuffle reduction).
 __device__ __forceinline__ void flush_probes(unsigned long long* out, const ProbeCount& pc) {
   if (!out) return;
@@ -74,7 +87,7 @@ __device__ __forceinline__ int own_packed(uint64_t* T, vid key, ST w, uint32_t H
     }
     h = probe_next(h, st, H);
   }
-  pc.add(H);
+  pc.over(H);
   return -1;
 }
 
@@ -94,7 +107,7 @@ __device__ __forceinline__ int own_sep(vid* K, ST* V, vid key, ST w, uint32_t H,
     }
     h = probe_next(h, st, H);
   }
-  pc.add(H);
+  pc.over(H);
   return -1;
 }
 

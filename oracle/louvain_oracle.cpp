@@ -80,12 +80,18 @@ static void vertex_weights(const G<W>& g, W* k) {
 }
 
 // ---------------------------------------------------------------------------
-// Move rule, one synchronous pass (Appendix: louvain.cu:454-585).
+// Move rule, one synchronous pass (louvain.cu:454-585).
 //   version 0/2: candidates = every neighbour community != own; argmax; clamp.
 //   version 1  : candidates filtered by direction before the argmax; no clamp.
+//   tie 0      : equal gains go to the smaller id on up passes and to the larger
+//                id on down passes (louvain.cu:544-554);
+//   tie 1      : equal gains always go to the smaller id (the deterministic
+//                smallest-id rule; no reference counterpart). The filter and
+//                the clamp keep the pass direction in both rules.
 template <class W>
 static void move_pass(const G<W>& g, const W* k, const i32* c, const W* a, W mass,
-                      int up, int version, i32* c_new) {
+                      int up, int version, int tie, i32* c_new) {
+  const bool small = tie ? true : (up != 0);
   const double M = (double)mass / 2.0;  // louvain.cu:454 "mass = mass / 2.0"
 #pragma omp parallel
   {
@@ -124,8 +130,8 @@ static void move_pass(const G<W>& g, const W* k, const i32* c, const W* a, W mas
         const double acj = (double)a[cj];
         wij -= (deici - dki * (daci - acj) / (2. * M));
         wij /= M;
-        bool better = up ? ((wij > best) || (wij == best && cj < dst))
-                         : ((wij > best) || (wij == best && cj > dst));  // :544-554
+        bool better = small ? ((wij > best) || (wij == best && cj < dst))
+                            : ((wij > best) || (wij == best && cj > dst));  // :544-554
         if (better) { dst = cj; best = wij; }
         acc[cj] = 0;
         seen[cj] = 0;
@@ -291,7 +297,7 @@ double modq<double>(const G<double>& g, const i32* c, const double* a, double ma
 }
 
 template <class W>
-static void louvain(CSR<W> in, double thr, int max_phases, int max_iter, int version,
+static void louvain(CSR<W> in, double thr, int max_phases, int max_iter, int version, int tie,
                     Result<W>& R) {
   W mass = 0;
   for (i64 e = 0; e < (i64)in.col.size(); ++e) mass += in.w.empty() ? (W)1 : in.w[e];
@@ -317,7 +323,7 @@ static void louvain(CSR<W> in, double thr, int max_phases, int max_iter, int ver
     int it = 0;
     while ((new_Q - cur_Q) > thr && (max_iter <= 0 || it < max_iter)) {
       cur_Q = new_Q;
-      move_pass(g, k.data(), c.data(), a.data(), mass, up ? 1 : 0, version, cn.data());
+      move_pass(g, k.data(), c.data(), a.data(), mass, up ? 1 : 0, version, tie, cn.data());
       up = !up;
       update_weights(cur.n, k.data(), c.data(), cn.data(), a.data());
       new_Q = modq(g, cn.data(), a.data(), mass);
@@ -388,8 +394,9 @@ static void copy_in(CSR<W>& g, i64 n, const i64* off, const i32* col) {
 }
 
 // mode 0: exact (w may be NULL = unit, or dyadic doubles); mode 1: fp64.
+// tie: 0 = reference rule (alternating with the pass direction), 1 = smallest id.
 extern "C" void* orc_louvain(i64 n, const i64* off, const i32* col, const double* w, int mode,
-                             double thr, int max_phases, int max_iter, int version) {
+                             double thr, int max_phases, int max_iter, int version, int tie) {
   ResultAny* R = new ResultAny();
   if (mode == 0) {
     R->exact = 1;
@@ -401,13 +408,13 @@ extern "C" void* orc_louvain(i64 n, const i64* off, const i32* col, const double
       g.w.resize(off[n]);
       for (i64 e = 0; e < off[n]; ++e) g.w[e] = (i64)std::ldexp(w[e], S);
     }
-    louvain(g, thr, max_phases, max_iter, version, R->ri);
+    louvain(g, thr, max_phases, max_iter, version, tie, R->ri);
   } else {
     R->exact = 0;
     CSR<double> g;
     copy_in(g, n, off, col);
     if (w) g.w.assign(w, w + off[n]);
-    louvain(g, thr, max_phases, max_iter, version, R->rd);
+    louvain(g, thr, max_phases, max_iter, version, tie, R->rd);
   }
   return R;
 }
@@ -444,9 +451,9 @@ extern "C" void orc_vertex_weights_i64(i64 n, const i64* off, const i64* w, i64*
 }
 extern "C" void orc_move_i64(i64 n, const i64* off, const i32* col, const i64* w, const i64* k,
                              const i32* c, const i64* a, i64 mass, int up, int version,
-                             i32* c_new) {
+                             int tie, i32* c_new) {
   G<i64> g{n, off, col, w};
-  move_pass(g, k, c, a, mass, up, version, c_new);
+  move_pass(g, k, c, a, mass, up, version, tie, c_new);
 }
 extern "C" void orc_update_weights_i64(i64 n, const i64* k, const i32* c, const i32* c_new,
                                        i64* a) {
@@ -460,9 +467,9 @@ extern "C" double orc_modularity_i64(i64 n, const i64* off, const i32* col, cons
 // fp64 step versions
 extern "C" void orc_move_f64(i64 n, const i64* off, const i32* col, const double* w,
                              const double* k, const i32* c, const double* a, double mass, int up,
-                             int version, i32* c_new) {
+                             int version, int tie, i32* c_new) {
   G<double> g{n, off, col, w};
-  move_pass(g, k, c, a, mass, up, version, c_new);
+  move_pass(g, k, c, a, mass, up, version, tie, c_new);
 }
 extern "C" double orc_modularity_f64(i64 n, const i64* off, const i32* col, const double* w,
                                      const i32* c, const double* a, double mass) {
@@ -491,7 +498,7 @@ extern "C" double orc_pass_i64(i64 n, const i64* off, const i32* col, const i64*
                                i32* c, i64* a, i64 mass, int up, int version, i32* c_new,
                                double cur_q, double thr) {
   G<i64> g{n, off, col, w};
-  move_pass(g, k, c, a, mass, up, version, c_new);
+  move_pass(g, k, c, a, mass, up, version, 0, c_new);
   update_weights(n, k, c, c_new, a);
   double q = modularity_exact(g, c_new, a, mass);
   if (q - cur_q > thr) std::memcpy(c, c_new, sizeof(i32) * n);
@@ -508,7 +515,7 @@ extern "C" double orc_pass_f64(i64 n, const i64* off, const i32* col, const doub
                                const double* k, i32* c, double* a, double mass, int up,
                                int version, i32* c_new, double cur_q, double thr) {
   G<double> g{n, off, col, w};
-  move_pass(g, k, c, a, mass, up, version, c_new);
+  move_pass(g, k, c, a, mass, up, version, 0, c_new);
   update_weights(n, k, c, c_new, a);
   double q = modularity_fp64(g, c_new, a, mass);
   if (q - cur_q > thr) std::memcpy(c, c_new, sizeof(i32) * n);

@@ -22,7 +22,7 @@ def _s(name, res, args):
     f.restype, f.argtypes = res, args
 
 
-_s("orc_louvain", _vp, [_i64, P64, P32, PD, _i, _d, _i, _i, _i])
+_s("orc_louvain", _vp, [_i64, P64, P32, PD, _i, _d, _i, _i, _i, _i])
 _s("orc_res_levels", _i, [_vp])
 _s("orc_res_n", _i64, [_vp, _i])
 _s("orc_res_q", _d, [_vp, _i])
@@ -39,10 +39,10 @@ _s("orc_res_free", None, [_vp])
 _s("orc_dyadic_scale", _i, [_i64, PD])
 _s("orc_u128_to_double", _d, [C.c_uint64, C.c_uint64])
 _s("orc_vertex_weights_i64", None, [_i64, P64, P64, P64])
-_s("orc_move_i64", None, [_i64, P64, P32, P64, P64, P32, P64, _i64, _i, _i, P32])
+_s("orc_move_i64", None, [_i64, P64, P32, P64, P64, P32, P64, _i64, _i, _i, _i, P32])
 _s("orc_update_weights_i64", None, [_i64, P64, P32, P32, P64])
 _s("orc_modularity_i64", _d, [_i64, P64, P32, P64, P32, P64, _i64])
-_s("orc_move_f64", None, [_i64, P64, P32, PD, PD, P32, PD, _d, _i, _i, P32])
+_s("orc_move_f64", None, [_i64, P64, P32, PD, PD, P32, PD, _d, _i, _i, _i, P32])
 _s("orc_modularity_f64", _d, [_i64, P64, P32, PD, P32, PD, _d])
 _s("orc_coarsen_i64", _i64, [_i64, P64, P32, P64, P32, P64, P32, P64, P32])
 _s("orc_pass_i64", _d, [_i64, P64, P32, P64, P64, P32, P64, _i64, _i, _i, P32, _d, _d])
@@ -64,15 +64,16 @@ class Result:
 
 
 def louvain(offsets, cols, w=None, mode=0, threshold=1e-6, max_phases=1000, max_iter=0,
-            version=2):
+            version=2, tie=0):
     """Full multi-level run. mode 0 = exact (unit or dyadic weights), 1 = fp64.
+    tie 0 = the reference's tie rule (alternating), 1 = always the smallest id.
     Returns levels [(n, q, iters, qtraj, assign)], graphs [(off, col, w)], final."""
     off = np.ascontiguousarray(offsets, np.int64)
     col = np.ascontiguousarray(cols, np.int32)
     wd = None if w is None else np.ascontiguousarray(w, np.float64)
     n = len(off) - 1
     h = lib.orc_louvain(n, _p(off, _i64), _p(col, _i32), None if wd is None else _p(wd, _d),
-                        mode, threshold, max_phases, max_iter, version)
+                        mode, threshold, max_phases, max_iter, version, tie)
     if not h:
         raise ValueError("oracle refused the weights for exact mode")
     R = Result()
@@ -92,7 +93,7 @@ def louvain(offsets, cols, w=None, mode=0, threshold=1e-6, max_phases=1000, max_
         if gm:
             lib.orc_res_graph_w(h, l, _p(gw, _d))
         R.graphs.append((go, gc, gw))
-    n0 = R.levels[0]["n"] if R.levels else 0
+    n0 = lib.orc_res_graph_n(h, 0)  # the input graph (kept as graphs[0])
     R.final = _arr(lib.orc_res_final(h), n0, np.int32)
     R.final_q = R.levels[-1]["q"] if R.levels else 0.0
     lib.orc_res_free(h)
@@ -117,7 +118,7 @@ def vertex_weights(off, w=None):
     return k[:n]
 
 
-def move(off, col, w, k, c, a, mass, up, version=2):
+def move(off, col, w, k, c, a, mass, up, version=2, tie=0):
     off = np.ascontiguousarray(off, np.int64)
     col = np.ascontiguousarray(col, np.int32)
     k = np.ascontiguousarray(k, np.int64)
@@ -126,7 +127,23 @@ def move(off, col, w, k, c, a, mass, up, version=2):
     n = len(off) - 1
     out = np.empty(max(n, 1), np.int32)
     lib.orc_move_i64(n, _p(off, _i64), _p(col, _i32), None if w is None else _p(w, _i64),
-                     _p(k, _i64), _p(c, _i32), _p(a, _i64), int(mass), int(up), version,
+                     _p(k, _i64), _p(c, _i32), _p(a, _i64), int(mass), int(up), version, tie,
+                     _p(out, _i32))
+    return out[:n]
+
+
+def move_f64(off, col, w, k, c, a, mass, up, version=2, tie=0):
+    """FP64 move pass (weights, totals and 2m as doubles)."""
+    off = np.ascontiguousarray(off, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    wd = None if w is None else np.ascontiguousarray(w, np.float64)
+    k = np.ascontiguousarray(k, np.float64)
+    c = np.ascontiguousarray(c, np.int32)
+    a = np.ascontiguousarray(a, np.float64)
+    n = len(off) - 1
+    out = np.empty(max(n, 1), np.int32)
+    lib.orc_move_f64(n, _p(off, _i64), _p(col, _i32), None if wd is None else _p(wd, _d),
+                     _p(k, _d), _p(c, _i32), _p(a, _d), float(mass), int(up), version, tie,
                      _p(out, _i32))
     return out[:n]
 

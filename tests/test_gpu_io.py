@@ -53,6 +53,59 @@ def test_mtx_general_weighted_through_gpu(ctx, mgl, orc, tmp_path):
     assert res.final_q == ref.final_q
 
 
+def _cli_tables(stdout):
+    """Per-phase rows (loop, Q, dQ, time s, time ms) of the reference's table
+    layout (louvain.cu:1536-1538, :1654)."""
+    phases, cur = [], None
+    for line in stdout.splitlines():
+        if line.startswith("Phase "):
+            cur = []
+            phases.append(cur)
+        elif line.startswith("| ") and cur is not None and not line.startswith("| Loop"):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            assert len(cells) == 5, line
+            # the reference prints "%-10d" / "%-10f": left-aligned in 10 columns
+            assert all(len(c) >= 1 for c in cells)
+            cur.append((int(cells[0]), float(cells[1]), float(cells[2]), float(cells[3]),
+                        float(cells[4])))
+    return phases
+
+
+def test_cli_table_layout(ctx, mgl, tmp_path):
+    # the path graph of the hand-derived known answer: per phase a loop-0 row
+    # with the singleton Q, then one row per iteration with new_Q and
+    # dQ = new_Q - cur_Q (0 after a rejected proposal), times per iteration
+    cli = os.path.join(ROOT, "mg-alphagcd_amd", "mgl_cli")
+    p = tmp_path / "path.mtx"
+    write_mtx(p, 5, [(2, 0), (4, 0), (2, 1), (4, 3)])
+    r = subprocess.run([cli, "-path_of_graph", str(p)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    print(r.stdout)
+    hdr = "| Loop       | Q          | dQ         | time(s)    | time(ms)   |"
+    assert hdr in r.stdout
+    assert "|------------|------------|------------|------------|------------|" in r.stdout
+    ph = _cli_tables(r.stdout)
+    assert len(ph) == 2
+    q = [(0, -7 / 32, 0.0), (1, 5 / 32, 12 / 32), (2, 7 / 32, 2 / 32), (3, 7 / 32, 0.0)]
+    assert [row[0] for row in ph[0]] == [0, 1, 2, 3]
+    for row, (loop, qq, dq) in zip(ph[0], q):
+        assert row[1] == pytest.approx(qq, abs=1e-6) and row[2] == pytest.approx(dq, abs=1e-6)
+    assert ph[0][0][3:] == (0.0, 0.0)
+    assert all(row[3] >= 0 and row[4] == pytest.approx(row[3] * 1000, abs=2e-3)
+               for row in ph[0][1:])
+    assert [row[0] for row in ph[1]] == [0, 1]
+    assert ph[1][0][1] == pytest.approx(7 / 32, abs=1e-6) and ph[1][1][2] == 0.0
+    for key in ("Main loop total execution time:", "The average execution time per loop:",
+                "Coarsen graph execution time:", "Total time for clustering   :",
+                "Total time for coarsen graph:", "TOTAL TIME                  :"):
+        assert key in r.stdout
+    # the level's Q trajectory and its singleton start through the library
+    res = ctx.louvain(mgl.read_mtx(str(p)))
+    assert res.levels[0].q0 == -7 / 32 and list(res.levels[0].qtraj) == [5 / 32, 7 / 32, 7 / 32]
+    assert len(res.levels[0].iter_seconds) == 3 and np.all(res.levels[0].iter_seconds > 0)
+
+
 def test_cli_labels_match_library(ctx, mgl, tmp_path):
     cli = os.path.join(ROOT, "mg-alphagcd_amd", "mgl_cli")
     out = tmp_path / "labels.txt"

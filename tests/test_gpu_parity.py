@@ -33,6 +33,26 @@ def compare(gpu, ref, check_graphs=True, scale=0):
     np.testing.assert_array_equal(gpu.final_assign, ref.final)
 
 
+def compare_fp64(gpu, ref, tol=1e-12):
+    """FP64 mode on dyadic weights: every sum of weights is exact, so the gains
+    and hence the assignments, iteration counts and coarse graphs are the
+    oracle's bit for bit; sum a^2 is not exact in FP64 and its rounding depends
+    on the summation order, so modularity values agree to tol."""
+    assert len(gpu.levels) == len(ref.levels), (len(gpu.levels), len(ref.levels))
+    for l, (g, r) in enumerate(zip(gpu.levels, ref.levels)):
+        assert g.n == r["n"] and g.iters == r["iters"], (l, g.n, r["n"], g.iters, r["iters"])
+        assert abs(g.q - r["q"]) <= tol
+        np.testing.assert_allclose(g.qtraj, r["qtraj"], rtol=0, atol=tol)
+        if g.assign is not None:
+            np.testing.assert_array_equal(g.assign, r["assign"])
+        if l >= 1 and g.graph is not None:
+            go, gc, gw = ref.graphs[l]
+            np.testing.assert_array_equal(g.graph.offsets, go)
+            np.testing.assert_array_equal(g.graph.cols, gc)
+            np.testing.assert_array_equal(g.graph_w64, gw)
+    np.testing.assert_array_equal(gpu.final_assign, ref.final)
+
+
 @pytest.mark.parametrize("version", [0, 1, 2])
 def test_two_triangles_known_answer(ctx, mgl, version):
     # hand-derived known answer (two triangles joined by edge 2-3)
@@ -149,39 +169,77 @@ def test_hub_path_giant_star(ctx, mgl, orc):
     compare(res, ref)
 
 
-def hub_graph(mgl):
+def hub_graph(mgl, weights=0):
     # R-MAT with a heavy tail: many rows in the hub bin
-    return mgl.gen_rmat(17, 48, seed=13)
+    return mgl.gen_rmat(17, 48, seed=13, weights=weights)
 
 
+HUB_TUNE_BINS = "".join(
+    f"{u} {sh} {s_} {sz}\n" for u, sh, s_, sz in
+    [(0, 0, 0, 0), (4, 1, 4, 3), (8, 1, 8, 4), (16, 1, 16, 5), (32, 1, 32, 6),
+     (64, 1, 64, 7), (128, 1, 64, 8), (256, 1, 64, 9), (1024, 2, 256, 2048),
+     (-1, 3, 0, 0)])
+
+
+class HubOracle:
+    """The oracle's move pass for the hub graph in one of three arithmetic
+    forms: unit weights, dyadic weights scaled to int64 (the exact mode), or
+    the same weights as doubles (the FP64 mode; every sum of these weights is
+    exact in FP64, so the two float orders agree bit for bit)."""
+
+    def __init__(self, orc, g, form):
+        self.orc, self.g, self.form = orc, g, form
+        if form == "unit":
+            self.w, self.mass = None, g.nnz
+            self.k = orc.vertex_weights(g.offsets)
+        elif form == "exact":
+            S = orc.scale_of(g.w)
+            self.w = orc.int_weights(g.w, S)
+            self.mass = int(self.w.sum())
+            self.k = orc.vertex_weights(g.offsets, self.w)
+        else:
+            self.w = g.w.astype(np.float64)
+            rows = np.repeat(np.arange(g.n), np.diff(g.offsets))
+            self.k = np.bincount(rows, weights=self.w, minlength=g.n)
+            self.mass = float(self.w.sum())
+
+    def move(self, c, up, version):
+        if self.form == "fp64":
+            a = np.zeros(self.g.n)
+            np.add.at(a, c, self.k)
+            return self.orc.move_f64(self.g.offsets, self.g.cols, self.w, self.k, c, a,
+                                     self.mass, up, version)
+        a = self.orc.community_totals(self.k, c, self.g.n)
+        return self.orc.move(self.g.offsets, self.g.cols, self.w, self.k, c, a, self.mass, up,
+                             version)
+
+
+@pytest.mark.parametrize("version", [0, 1, 2])
 @pytest.mark.parametrize("segmented", [1, 0])
 @pytest.mark.parametrize("bucket", [4096, 1 << 20])
-def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, bucket, segmented):
+@pytest.mark.parametrize("form", ["unit", "exact", "fp64"])
+def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, form, bucket, segmented, version):
     # bucket = 2^20 makes each hub a single bucket larger than the LDS table,
     # which exercises the HBM-table fallback of the aggregate step;
-    # segmented = 0 forces the scatter form of the hub path
-    g = hub_graph(mgl)
+    # segmented = 0 forces the scatter form of the hub path; the weighted forms
+    # take the 8-byte-sum tables (ModeInt, ModeF64)
+    g = hub_graph(mgl, weights=0 if form == "unit" else 1)
     tf = tmp_path / "tune.txt"
-    tf.write_text(f"hub_bucket_unclamped {bucket}\nhub_segmented {segmented}\n" + "".join(
-        f"{u} {sh} {s_} {sz}\n" for u, sh, s_, sz in
-        [(0, 0, 0, 0), (4, 1, 4, 3), (8, 1, 8, 4), (16, 1, 16, 5), (32, 1, 32, 6),
-         (64, 1, 64, 7), (128, 1, 64, 8), (256, 1, 64, 9), (1024, 2, 256, 2048),
-         (-1, 3, 0, 0)]))
-    s = ctx.session(g, tuning_file=str(tf))
+    tf.write_text(f"hub_bucket_unclamped {bucket}\nhub_segmented {segmented}\n" + HUB_TUNE_BINS)
+    prec = {"unit": mgl.PREC_EXACT, "exact": mgl.PREC_EXACT, "fp64": mgl.PREC_FP64}[form]
+    s = ctx.session(g, tuning_file=str(tf), version=version, precision=prec)
     bins = s.bins()
-    print("bins", bins, flush=True)
+    print(f"{form} v{version} bins {bins}", flush=True)
     assert bins[-1] > 10
-    k = orc.vertex_weights(g.offsets)
+    ho = HubOracle(orc, g, form)
     rng = np.random.default_rng(5)
     for trial in range(3):
         c = rng.integers(0, g.n, g.n).astype(np.int32)
         if trial > 0:
             c = (c // (7 * trial)) * (7 * trial)
         s.set_assign(c)
-        a = orc.community_totals(k, c, g.n)
         for up in (True, False):
-            np.testing.assert_array_equal(
-                s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
+            np.testing.assert_array_equal(s.move(up), ho.move(c, up, version))
     s.close()
 
 
@@ -238,6 +296,10 @@ _GOOD_BINS = "0 0 0 0\n32 1 32 6\n-1 3 0 0\n"
     ("0 0 0 0\n32 1 32 6\n", "hub path"),
     ("4 4 1 0\n-1 3 0 0\n", "degree 0"),
     ("0 0 0 0\n64 1 32 5\n-1 3 0 0\n", "table too small"),
+    ("0 0 0 0\n16 1 0 5\n-1 3 0 0\n", "lanes per vertex"),
+    ("0 0 0 0\n16 1 512 5\n-1 3 0 0\n", "lanes per vertex"),
+    ("0 0 0 0\n16 1 4 6\n-1 3 0 0\n", "no compiled kernel"),
+    ("0 0 0 0\n512 2 128 600\n-1 3 0 0\n", "threads"),
 ])
 def test_tuning_file_errors(ctx, mgl, tmp_path, text, reason):
     # every malformed tuning file is refused with INVALID_ARG naming the
@@ -259,11 +321,28 @@ def test_tuning_file_errors(ctx, mgl, tmp_path, text, reason):
     s.close()
 
 
-def test_hub_graph_full_parity(ctx, mgl, orc):
-    g = hub_graph(mgl)
-    res = ctx.louvain(g, keep_levels=True, keep_graphs=True)
-    ref = orc.louvain(g.offsets, g.cols, None)
-    compare(res, ref)
+@pytest.mark.parametrize("version", [0, 1, 2])
+@pytest.mark.parametrize("form", ["unit", "exact", "fp64"])
+def test_hub_graph_full_parity(ctx, mgl, orc, form, version):
+    g = hub_graph(mgl, weights=0 if form == "unit" else 1)
+    print(f"hub graph {form} v{version} n={g.n} nnz={g.nnz}", flush=True)
+    if form == "unit":
+        res = ctx.louvain(g, version=version, keep_levels=True, keep_graphs=True)
+        compare(res, orc.louvain(g.offsets, g.cols, None, version=version))
+    elif form == "exact":
+        S = orc.scale_of(g.w)
+        res = ctx.louvain(g, version=version, precision=mgl.PREC_EXACT, keep_levels=True,
+                          keep_graphs=True)
+        compare(res, orc.louvain(g.offsets, g.cols, g.w.astype(np.float64), mode=0,
+                                 version=version), scale=S)
+    else:
+        # dyadic weights whose sums are exact in FP64: the FP64 mode is bit-equal
+        # to the oracle's FP64 mode
+        res = ctx.louvain(g, version=version, precision=mgl.PREC_FP64, keep_levels=True,
+                          keep_graphs=True)
+        assert res.precision == mgl.PREC_FP64
+        compare_fp64(res, orc.louvain(g.offsets, g.cols, g.w.astype(np.float64), mode=1,
+                                      version=version))
 
 
 def test_edge_cases(ctx, mgl, orc):
@@ -296,69 +375,12 @@ def test_sbm_weighted_exact(ctx, mgl, orc):
     compare(res, ref, scale=S)
 
 
-@pytest.mark.parametrize("graph", [True, False])
-def test_batched_equals_plain(ctx, mgl, graph):
-    graphs = [mgl.gen_rmat(10 + (i % 3), 8 + 4 * (i % 3), seed=100 + i) for i in range(5)]
-    graphs.append(two_triangles(mgl))
-    graphs.append(mgl.gen_sbm(3000, 12, min_size=64, seed=11, weights=0, deg_in=16, deg_out=3))
-    batch = ctx.louvain_batched(graphs, keep_levels=True, use_graph=graph)
-    for g, b in zip(graphs, batch):
-        p = ctx.louvain(g, keep_levels=True, use_graph=False)
-        assert len(b.levels) == len(p.levels)
-        for lb, lp in zip(b.levels, p.levels):
-            assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
-            np.testing.assert_array_equal(lb.qtraj, lp.qtraj)
-            np.testing.assert_array_equal(lb.assign, lp.assign)
-        np.testing.assert_array_equal(b.final_assign, p.final_assign)
-
-
-def test_batched_weighted_and_unit_graphs(ctx, mgl):
-    # unit-weight graphs inside a batch with FP32 weights run in the batch's
-    # fixed-point scale (each unit weight is 2^S); their 2m must be scaled too
-    graphs = [mgl.gen_rmat(11, 24, seed=900, weights=1), mgl.gen_rmat(12, 25, seed=901),
-              mgl.gen_sbm(3000, 12, min_size=64, seed=902, weights=0, deg_in=16, deg_out=3),
-              mgl.gen_sbm(2000, 8, min_size=64, seed=903, weights=1, deg_in=16, deg_out=3)]
-    batch = ctx.louvain_batched(graphs, keep_levels=True)
-    for g, b in zip(graphs, batch):
-        p = ctx.louvain(g, keep_levels=True)
-        assert len(b.levels) == len(p.levels)
-        for lb, lp in zip(b.levels, p.levels):
-            assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
-            np.testing.assert_array_equal(lb.assign, lp.assign)
-        np.testing.assert_array_equal(b.final_assign, p.final_assign)
-
-
 def test_repeat_runs_identical(ctx, mgl):
     g = mgl.gen_rmat(15, 16, seed=21)
     a = ctx.louvain(g, keep_levels=True)
     b = ctx.louvain(g, keep_levels=True)
     assert a.final_q == b.final_q
     np.testing.assert_array_equal(a.final_assign, b.final_assign)
-
-
-@pytest.mark.parametrize("graph", [True, False])
-@pytest.mark.parametrize("fused", [True, False])
-def test_batched_weighted_mixed_sizes(ctx, mgl, graph, fused):
-    # graphs of very different sizes finish at different iterations and levels,
-    # which exercises the batch compaction and per-graph coarsening
-    graphs = []
-    for i in range(24):
-        if i % 2 == 0:
-            graphs.append(mgl.gen_sbm(500 + 300 * i, 4 + i, min_size=8, deg_in=16, deg_out=3,
-                                      seed=200 + i, weights=1))
-        else:
-            graphs.append(mgl.gen_rmat(9 + (i % 5), 8 + i, seed=300 + i, weights=1,
-                                       n_limit=400 + 250 * i))
-    batch = ctx.louvain_batched(graphs, keep_levels=True, use_graph=graph,
-                                fuse_modularity=fused)
-    for g, b in zip(graphs, batch):
-        p = ctx.louvain(g, keep_levels=True, use_graph=False, fuse_modularity=fused)
-        assert len(b.levels) == len(p.levels)
-        for lb, lp in zip(b.levels, p.levels):
-            assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
-            np.testing.assert_array_equal(lb.qtraj, lp.qtraj)
-            np.testing.assert_array_equal(lb.assign, lp.assign)
-        np.testing.assert_array_equal(b.final_assign, p.final_assign)
 
 
 # ---- records in degree-bin order (forced on graphs below the automatic size) ----
@@ -402,29 +424,6 @@ def test_binned_layout_move_stage(ctx, mgl, orc):
                 s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
         assert s.modularity() == orc.modularity(g.offsets, g.cols, None, c, a, g.nnz)
     s.close()
-
-
-@pytest.mark.parametrize("fused", [True, False])
-def test_binned_layout_batched(ctx, mgl, fused):
-    # a batch with compaction (graphs finish at different iterations)
-    graphs = []
-    for i in range(12):
-        if i % 2 == 0:
-            graphs.append(mgl.gen_sbm(600 + 400 * i, 4 + i, min_size=8, deg_in=16, deg_out=3,
-                                      seed=500 + i, weights=1))
-        else:
-            graphs.append(mgl.gen_rmat(9 + (i % 5), 8 + i, seed=600 + i, weights=1,
-                                       n_limit=500 + 300 * i))
-    batch = ctx.louvain_batched(graphs, keep_levels=True, fuse_modularity=fused,
-                                record_layout=mgl.LAYOUT_BINNED)
-    for g, b in zip(graphs, batch):
-        p = ctx.louvain(g, keep_levels=True, fuse_modularity=fused,
-                        record_layout=mgl.LAYOUT_VERTEX)
-        assert len(b.levels) == len(p.levels)
-        for lb, lp in zip(b.levels, p.levels):
-            assert lb.n == lp.n and lb.q == lp.q and lb.iters == lp.iters
-            np.testing.assert_array_equal(lb.assign, lp.assign)
-        np.testing.assert_array_equal(b.final_assign, p.final_assign)
 
 
 def test_autotune_layout_parity(ctx, mgl, orc, tmp_path):

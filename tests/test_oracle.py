@@ -2,8 +2,11 @@
 retype its formulas:
 
   * move rule     : exact-rational dQ by full modularity recomputation on tiny
-                    random graphs (fractions), for v0/v1/v2 and both directions
-  * known answer  : two triangles joined by an edge (hand-derived per pass)
+                    random graphs (fractions), for v0/v1/v2, both directions and
+                    both tie rules; on graphs whose 2m is a power of two every
+                    FP64 gain is exact, so ties and zero gains are checked too
+  * known answers : two triangles joined by an edge; a path whose run has a
+                    down-pass tie and a zero-gain move (hand-derived per pass)
   * modularity    : closed forms (one community, singletons, ring of cliques)
                     and networkx.community.modularity (fixture + live)
   * coarsening    : dict-based brute force, weight conservation, symmetry,
@@ -95,6 +98,148 @@ def test_move_rule_brute_force(orc, version):
                 assert got[v] == dst, (trial, v, up, gains, got[v])
                 checked += 1
     assert checked > 200
+
+
+def q_csr(off, col, w, c):
+    """Modularity of assignment c on a CSR (directed entries, self-loops once)
+    with fractions: sum of w over entries inside a community / 2m minus the
+    squared community totals (row sums) / (2m)^2."""
+    n = len(off) - 1
+    wt = [Fraction(1)] * len(col) if w is None else [Fraction(x) for x in w]
+    m2 = sum(wt)
+    k = [sum(wt[off[v]:off[v + 1]], Fraction(0)) for v in range(n)]
+    sin = sum((wt[e] for v in range(n) for e in range(off[v], off[v + 1])
+               if c[v] == c[col[e]]), Fraction(0))
+    tot = {}
+    for v in range(n):
+        tot[c[v]] = tot.get(c[v], 0) + k[v]
+    return sin / m2 - sum(t * t for t in tot.values()) / (m2 * m2)
+
+
+def rule_proposal(c, gains, up, version, tie):
+    """The move rule applied to exact gains {v: {community: gain}}: start from
+    (0, own community), higher gain wins, equal gains by id (the smaller on up
+    passes or always with tie=1, else the larger); v1 filters candidates by
+    direction first, v0/v2 clamp the result to the direction."""
+    out = []
+    for v, ci in enumerate(c):
+        best, dst = Fraction(0), ci
+        small = True if tie else up
+        for x, gv in sorted(gains[v].items()):
+            if version == 1 and not (x < ci if up else x > ci):
+                continue
+            if gv > best or (gv == best and (x < dst if small else x > dst)):
+                best, dst = gv, x
+        if version != 1:
+            dst = min(dst, ci) if up else max(dst, ci)
+        out.append(dst)
+    return out
+
+
+def exact_gains(off, col, w, c):
+    """Delta-Q of moving each vertex alone into each neighbouring community."""
+    q0 = q_csr(off, col, w, c)
+    out = []
+    for v in range(len(off) - 1):
+        g = {}
+        for e in range(off[v], off[v + 1]):
+            x = int(c[col[e]])
+            if x != c[v] and x not in g:
+                c2 = list(c)
+                c2[v] = x
+                g[x] = q_csr(off, col, w, c2) - q0
+        out.append(g)
+    return out
+
+
+def test_known_answer_path_ties(orc):
+    with open(os.path.join(GOLDEN, "kat_path_ties.json")) as f:
+        kat = json.load(f)
+    g = csr_from_edges(kat["n"], kat["edges"])
+    assert g.nnz == kat["two_m"]
+    for rule in kat["rules"].values():
+        # the derivation itself, from full modularity recomputations
+        graphs = [(list(g.offsets), list(g.cols), None)]
+        for lev in rule["levels"][1:]:
+            gg = lev["graph"]
+            graphs.append((gg["offsets"], gg["cols"], gg["w"]))
+        for li, lev in enumerate(rule["levels"]):
+            off, col, w = graphs[li]
+            ident = list(range(lev["n"]))
+            assert q_csr(off, col, w, ident) == Fraction(lev["q0"])
+            for ps in lev["passes"]:
+                ex = exact_gains(off, col, w, ps["c"])
+                listed = [{int(x): Fraction(y) for x, y in ps["gains"][str(v)].items()}
+                          for v in range(lev["n"])]
+                assert ex == listed
+                for ver in rule["versions"]:
+                    assert rule_proposal(ps["c"], listed, ps["up"], ver, rule["tie"]) == \
+                        ps["proposal"]
+                assert q_csr(off, col, w, ps["proposal"]) == Fraction(ps["q"])
+            if li + 1 < len(rule["levels"]):
+                # the next level's graph is the coarsening by the committed assignment
+                o2, c2, w2, _ = orc.coarsen(np.asarray(off), np.asarray(col),
+                                            None if w is None else np.asarray(w, np.int64),
+                                            np.asarray(lev["assign"], np.int32))
+                nxt = rule["levels"][li + 1]["graph"]
+                assert list(o2) == nxt["offsets"] and list(c2) == nxt["cols"]
+                assert list(w2) == nxt["w"]
+        # the oracle against the derivation
+        for ver in rule["versions"]:
+            R = orc.louvain(g.offsets, g.cols, None, version=ver, tie=rule["tie"])
+            assert len(R.levels) == len(rule["levels"])
+            for li, (lv, le) in enumerate(zip(R.levels, rule["levels"])):
+                assert lv["n"] == le["n"] and lv["iters"] == le["iters"]
+                assert list(lv["qtraj"]) == [float(Fraction(x)) for x in le["qtraj"]]
+                np.testing.assert_array_equal(lv["assign"], le["assign"])
+                if li >= 1:
+                    go, gc, gw = R.graphs[li]
+                    assert list(go) == le["graph"]["offsets"] and list(gc) == le["graph"]["cols"]
+                    assert list(gw) == le["graph"]["w"]
+            np.testing.assert_array_equal(R.final, rule["final"])
+            assert R.final_q == float(Fraction(rule["final_q"]))
+            # and pass by pass through the single-step entry point
+            k = orc.vertex_weights(g.offsets)
+            for ps in rule["levels"][0]["passes"]:
+                c = np.asarray(ps["c"], np.int32)
+                a = orc.community_totals(k, c, kat["n"])
+                got = orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, ps["up"], ver,
+                               rule["tie"])
+                np.testing.assert_array_equal(got, ps["proposal"])
+
+
+@pytest.mark.parametrize("tie", [0, 1])
+@pytest.mark.parametrize("version", [0, 1, 2])
+def test_move_rule_exact_ties(orc, version, tie):
+    # graphs with 2m in {8, 16, 32}: every division in the FP64 gain is by a
+    # power of two, so the gains are exact and equal gains are equal doubles;
+    # no row is skipped, ties and zero gains included
+    rng = np.random.default_rng(77 + 10 * version + tie)
+    checked = ties = zeros = 0
+    for trial in range(150):
+        n = int(rng.integers(4, 10))
+        pairs = [(u, v) for u in range(n) for v in range(u + 1, n)]
+        ne = int(rng.choice([4, 8, 16]))
+        if ne > len(pairs):
+            continue
+        idx = rng.choice(len(pairs), ne, replace=False)
+        edges = [pairs[i] for i in idx]
+        g = csr_from_edges(n, edges)
+        # coarse random assignments produce equal community totals often
+        c = (rng.integers(0, n, n) // int(rng.integers(1, 3))).astype(np.int32)
+        k = orc.vertex_weights(g.offsets)
+        a = orc.community_totals(k, c, n)
+        ex = exact_gains(list(g.offsets), list(g.cols), None, list(c))
+        for up in (True, False):
+            got = orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up, version, tie)
+            want = rule_proposal(list(c), ex, up, version, tie)
+            assert list(got) == want, (trial, up, ex)
+            checked += n
+            for gv in ex:
+                vals = list(gv.values())
+                ties += sum(1 for x in set(vals) if vals.count(x) > 1)
+                zeros += vals.count(0)
+    assert checked > 1000 and ties > 20 and zeros > 20, (checked, ties, zeros)
 
 
 def test_known_answer_two_triangles(orc):
