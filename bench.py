@@ -17,10 +17,11 @@ rmat (default): R-MAT scale 24, edge factor 32, (a,b,c,d) = (0.57, 0.19, 0.19,
 batched: 4,096 independent graphs, vertex counts log-uniform in [1,000,
   200,000] (seed 4,000,000 + i); even i planted partition (8-256 communities,
   intra / inter mean degree 16 / 3), odd i R-MAT (edge factor 8-32); weights
-  uniform (0.5, 1.5]. A step is one mgl_louvain_batched call on --batch graphs
-  (full multi-level Louvain, inner loops captured as hipGraphs); the plain form
-  (mgl_louvain per graph) runs on the same graphs and every result is compared.
-  value = graphs/s of the batched form.
+  uniform (0.5, 1.5]. A step is one mgl_louvain_batched call over all the
+  graphs (full multi-level Louvain, inner loops captured as hipGraphs, the
+  batch coarsened in groups of graphs); the plain form (mgl_louvain per graph)
+  runs on the same graphs and every result is compared. value = graphs/s of
+  the batched form.
 
 mixed: R-MAT scale 25, edge factor 16, seed 5, lognormal(0, 1) edge weights
   stored as FP32. A step is one level-0 pass in the FP32-hash mode (FP32 hash
@@ -398,40 +399,47 @@ def make_batch_graph(mgl, i):
 
 
 def run_batched(args, mgl):
-    ctx = mgl.Context(0)
-    total = args.steps * args.batch
     t0 = time.perf_counter()
-    sample = []  # the first graphs, kept for the CPU baseline
-    total_v = total_e = 0
-    bat_s = plain_s = 0.0
-    plain_n = mismatches = 0
-    for b0 in range(0, total, args.batch):
-        gs = [make_batch_graph(mgl, i) for i in range(b0, min(total, b0 + args.batch))]
-        if b0 == 0:
-            sample = gs[:64]
-            for _ in range(args.warmup):  # not timed, not counted
-                ctx.louvain_batched(gs[:64])
-        total_v += sum(g.n for g in gs)
-        total_e += sum(g.nnz for g in gs)
+    total = args.graphs
+    gs = [make_batch_graph(mgl, i) for i in range(total)]
+    total_v = sum(g.n for g in gs)
+    total_e = sum(g.nnz for g in gs)
+    log(f"{total} graphs: {total_v} vertices, {total_e} entries "
+        f"({time.perf_counter() - t0:.0f} s to generate)")
+    ctx = mgl.Context(0)
+    for _ in range(args.warmup):  # not timed
+        ctx.louvain_batched(gs[:64])
+    mgl.memory_stats(reset_peak=True)
+    bat_s = 0.0
+    res = None
+    for step in range(args.steps):
+        # one mgl_louvain_batched call over every graph of the workload
         res = ctx.louvain_batched(gs)
         bat_s += res[0].seconds
-        for j, (g, r) in enumerate(zip(gs, res)):
-            if args.plain_sample and (b0 + j) % args.plain_sample:
-                continue
-            p = ctx.louvain(g)
-            plain_s += p.seconds
-            plain_n += 1
-            if (p.final_q != r.final_q or len(p.levels) != len(r.levels)
-                    or not np.array_equal(p.final_assign, r.final_assign)):
-                mismatches += 1
-        log(f"graphs {b0}..{b0 + len(gs) - 1}: batched {res[0].seconds:.3f} s, plain so far "
-            f"{plain_s:.3f} s over {plain_n} graphs ({time.perf_counter() - t0:.0f} s elapsed)")
+        log(f"step {step + 1}: one batched call over {total} graphs: {res[0].seconds:.3f} s")
+    live, peak = mgl.memory_stats()
+    log(f"device memory held by the library: peak {peak / 2**30:.1f} GiB")
+    plain_s = 0.0
+    plain_n = mismatches = 0
+    for i, (g, r) in enumerate(zip(gs, res)):
+        if args.plain_sample and i % args.plain_sample:
+            continue
+        p = ctx.louvain(g)
+        plain_s += p.seconds
+        plain_n += 1
+        if (p.final_q != r.final_q or len(p.levels) != len(r.levels)
+                or not np.array_equal(p.final_assign, r.final_assign)
+                or [l.iters for l in p.levels] != [l.iters for l in r.levels]):
+            mismatches += 1
+        if i % 512 == 511:
+            log(f"plain: {plain_n} graphs, {plain_s:.3f} s ({time.perf_counter() - t0:.0f} s "
+                "elapsed)")
     ctx.close()
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_batched(sample, args.cpu_seconds)
+        cpu = cpu_baseline_batched(gs[:64], args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
-    gps_b = total / bat_s
+    gps_b = total * args.steps / bat_s
     gps_p = plain_n / plain_s
     return {
         "metric": METRIC,
@@ -448,10 +456,9 @@ def run_batched(args, mgl):
         "data": "synthetic",
         "config": {
             "workload": f"{total} graphs (1k-200k vertices log-uniform, half planted partition, "
-                        "half R-MAT, weights (0.5,1.5]), full multi-level Louvain, batches of "
-                        f"{args.batch} in one launch sequence each",
+                        "half R-MAT, weights (0.5,1.5]), full multi-level Louvain, all graphs in "
+                        "one mgl_louvain_batched call per step",
             "graphs": total,
-            "batch": args.batch,
             "vertices": int(total_v),
             "directed_entries": int(total_e),
             "precision": "exact (dyadic float weights)",
@@ -462,6 +469,7 @@ def run_batched(args, mgl):
         "batched_seconds": bat_s,
         "plain_seconds": plain_s,
         "mismatches": mismatches,
+        "device_peak_bytes": peak,
         "roofline": None,
         "cpu_baseline": cpu,
     }
@@ -508,6 +516,9 @@ def run_mixed(args, mgl):
     log(f"graph: R-MAT scale {args.scale} ef {args.edge_factor}: n={g.n} nnz={g.nnz} "
         f"lognormal weights ({time.perf_counter() - t0:.1f} s)")
     ctx = mgl.Context(0)
+    if args.membench:
+        copy_gbps = ctx.membench(args.membench, 5)
+        log(f"streaming copy of {args.membench} bytes: {copy_gbps:.1f} GB/s (read + write)")
     sess = ctx.session(g, precision=mgl.PREC_FP32_HASH, record_layout=args.record_layout)
     sess.reset()
     stats, wall = timed_passes(sess, args.steps, args.warmup)
@@ -517,6 +528,13 @@ def run_mixed(args, mgl):
         f"rmat{args.scale}_ef{args.edge_factor:g}_seed{args.seed}_lognormal_fp32")
     runs = {}
     modes = {}
+    if args.no_full:  # timed passes only (counter profiles)
+        ctx.close()
+        out = {"metric": METRIC, "value": g.nnz * args.steps / wall, "unit": "edges/s",
+               "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+               "ms_per_step": ms_per_step, "roofline": roof}
+        out.update(extra)
+        return out
     for name, prec in (("fp32_hash", mgl.PREC_FP32_HASH), ("fp64", mgl.PREC_FP64)):
         r = ctx.louvain(g, precision=prec, record_layout=args.record_layout)
         runs[name] = r
@@ -572,8 +590,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", choices=["rmat", "batched", "mixed"], default="rmat")
     ap.add_argument("--steps", type=int, default=None,
-                    help="timed steps (rmat/mixed: passes, default 10; batched: batches, "
-                         "default 4)")
+                    help="timed steps (rmat/mixed: passes, default 10; batched: calls over "
+                         "all graphs, default 1)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--scale", type=int, default=None)
     ap.add_argument("--edge-factor", type=float, default=None)
@@ -588,6 +606,8 @@ def main():
                     help="batched: CPU-baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--full", action="store_true", help="rmat: also time the full run")
+    ap.add_argument("--no-full", action="store_true",
+                    help="mixed: timed passes only, no full runs (counter profiles)")
     ap.add_argument("--tuning-file", default=None)
     ap.add_argument("--stats-out", default=None, help="write per-pass statistics (JSON)")
     ap.add_argument("--membench", type=int, default=0,
@@ -595,7 +615,7 @@ def main():
     ap.add_argument("--autotune", default=None,
                     help="run the bin-layout autotuner on this graph, write the layout to "
                          "this path and time with it")
-    ap.add_argument("--batch", type=int, default=1024, help="batched: graphs per launch")
+    ap.add_argument("--graphs", type=int, default=4096, help="batched: graphs of the workload")
     ap.add_argument("--plain-sample", type=int, default=0,
                     help="batched: time the plain form on every k-th graph only (0 = all)")
     args = ap.parse_args()
@@ -607,7 +627,7 @@ def main():
     # level-0 passes of the R-MAT workloads (scale 24 ef 32: 17 iterations;
     # scale 25 ef 16 lognormal: 11, the last one rejected); a timed pass that
     # ends the level is followed by an untimed restart
-    defaults = {"rmat": (24, 32, 3, 10), "mixed": (25, 16, 5, 8), "batched": (0, 0, 0, 4)}
+    defaults = {"rmat": (24, 32, 3, 10), "mixed": (25, 16, 5, 8), "batched": (0, 0, 0, 1)}
     sc, ef, sd, st = defaults[args.workload]
     args.scale = sc if args.scale is None else args.scale
     args.edge_factor = ef if args.edge_factor is None else args.edge_factor

@@ -14,8 +14,9 @@
  *
  * Conventions: plain pointers and sizes, no exceptions, no exit(); every call
  * returns an mgl_status and the context keeps the last error string. One
- * context = one device + one stream; a context is not thread safe, distinct
- * contexts may be used from distinct threads. Inputs are borrowed for the
+ * context = one device and its two streams (the move pass runs its hub path on
+ * the second); a context is not thread safe, distinct contexts may be used from
+ * distinct threads. Inputs are borrowed for the
  * duration of a call; results are owned by the library until freed.
  */
 #ifndef MGL_H
@@ -84,7 +85,8 @@ typedef struct {
 
 /* Record layout of a level (results are identical in every layout). */
 enum {
-  MGL_LAYOUT_AUTO = 0,   /* degree-bin order for levels of more than 2^19 vertices, else vertex order */
+  MGL_LAYOUT_AUTO = 0,   /* degree-bin order when the level's largest graph has more than
+                            2^19 vertices, else vertex order */
   MGL_LAYOUT_VERTEX = 1, /* record of v at position v */
   MGL_LAYOUT_BINNED = 2  /* records in degree-bin order: hub records share cache lines */
 };
@@ -266,6 +268,10 @@ int mgl_version_major(void);
 int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins);
 /* LDS bytes one workgroup may use on the context's device (sizes the tables). */
 int mgl_device_lds_bytes(const mgl_ctx* ctx);
+/* Device memory the library's calls hold now and held at most (process-wide,
+ * all devices), in bytes; reset_peak = 1 restarts the peak from the current
+ * value. */
+mgl_status mgl_memory_stats(int64_t* live_bytes, int64_t* peak_bytes, int reset_peak);
 /* Copies `bytes` between two device buffers `iters` times with a streaming
  * copy kernel and returns the best read+write GB/s (HBM calibration). */
 mgl_status mgl_membench(mgl_ctx* ctx, int64_t bytes, int iters, double* gbps);

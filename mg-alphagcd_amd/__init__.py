@@ -143,6 +143,7 @@ _sig("mgl_bin_bounds", _i, [_vp, C.POINTER(_i64), _i])
 _sig("mgl_autotune", _i, [_vp, C.POINTER(_Csr), C.c_char_p])
 _sig("mgl_device_lds_bytes", _i, [_vp])
 _sig("mgl_membench", _i, [_vp, _i64, _i, C.POINTER(_d)])
+_sig("mgl_memory_stats", _i, [C.POINTER(_i64), C.POINTER(_i64), _i])
 _sig("mgl_modularity_dev", _i, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, C.POINTER(_d)])
 
 
@@ -244,6 +245,13 @@ def gen_sbm(n: int, n_comm: int, exponent=2.1, min_size=64, deg_in=24.0, deg_out
     _chk(_lib.mgl_gen_sbm(n, n_comm, exponent, min_size, deg_in, deg_out, seed, weights,
                           C.byref(h)))
     return _take_host_csr(h)
+
+
+def memory_stats(reset_peak: bool = False):
+    """(live, peak) bytes of device memory held by the library's calls."""
+    live, peak = C.c_int64(), C.c_int64()
+    _chk(_lib.mgl_memory_stats(C.byref(live), C.byref(peak), int(reset_peak)))
+    return live.value, peak.value
 
 
 def gen_batch_graph(i: int) -> CSR:

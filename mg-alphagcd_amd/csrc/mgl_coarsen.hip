@@ -303,10 +303,14 @@ __global__ void k_newc(int64_t n, const vid* __restrict__ c, const int32_t* __re
   }
 }
 
-// Keys of every entry of a kept row, written at e + delta[g] (compaction of
-// the kept graphs' entry ranges). Entry-parallel: a wavefront takes 256
-// consecutive entries, finds the rows spanning them once, and each lane finds
-// its entry's row by binary search over that short range of offsets.
+// Keys of every entry of a kept row of the group (rows [v_lo, v_hi), entries
+// [e_lo, e_hi)), written at e + delta[g] - kbase (compaction of the kept
+// graphs' entry ranges; kbase = the group's first compacted position). The
+// key holds the coarse row and column relative to the group's first coarse
+// row r_lo (every entry of a graph stays inside its graph, hence inside its
+// group). Entry-parallel: a wavefront takes 256 consecutive entries, finds the
+// rows spanning them once, and each lane finds its entry's row by binary
+// search over that short range of offsets.
 __device__ __forceinline__ int64_t row_of(const eid* __restrict__ off, int64_t lo, int64_t hi,
                                           int64_t e) {
   // largest r in [lo, hi] with off[r] <= e
@@ -318,19 +322,21 @@ __device__ __forceinline__ int64_t row_of(const eid* __restrict__ off, int64_t l
 }
 
 template <class M, class VT>
-__global__ __launch_bounds__(256) void k_keys(int64_t n, int64_t nnz, const eid* __restrict__ off,
+__global__ __launch_bounds__(256) void k_keys(int64_t v_lo, int64_t v_hi, int64_t e_lo,
+                                              int64_t e_hi, const eid* __restrict__ off,
                                               const vid* __restrict__ col, const void* w,
                                               const vid* __restrict__ newc,
                                               const int32_t* __restrict__ gid,
-                                              const int64_t* __restrict__ delta, int bits,
-                                              uint64_t* __restrict__ keys, VT* __restrict__ vals) {
+                                              const int64_t* __restrict__ delta, int64_t kbase,
+                                              int64_t r_lo, int bits, uint64_t* __restrict__ keys,
+                                              VT* __restrict__ vals) {
   const int64_t wave = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
-  const int64_t e0 = wave * 256;
-  if (e0 >= nnz) return;
-  const int64_t e1 = (e0 + 256 < nnz) ? e0 + 256 : nnz;
+  const int64_t e0 = e_lo + wave * 256;
+  if (e0 >= e_hi) return;
+  const int64_t e1 = (e0 + 256 < e_hi) ? e0 + 256 : e_hi;
   int64_t r = 0;
-  if (lane < 2) r = row_of(off, 0, n - 1, lane == 0 ? e0 : e1 - 1);
+  if (lane < 2) r = row_of(off, v_lo, v_hi - 1, lane == 0 ? e0 : e1 - 1);
   const int64_t rlo = bcast64(r, 0), rhi = bcast64(r, 1);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -339,11 +345,17 @@ __global__ __launch_bounds__(256) void k_keys(int64_t n, int64_t nnz, const eid*
     const int64_t v = row_of(off, rlo, rhi, e);
     const vid rr = newc[v];
     if (rr < 0) continue;
-    const int64_t dl = delta ? delta[gid ? gid[v] : 0] : 0;
-    const uint64_t cc = (uint64_t)(uint32_t)newc[col[e]];
-    keys[e + dl] = ((uint64_t)(uint32_t)rr << bits) | cc;
+    const int64_t dl = (delta ? delta[gid ? gid[v] : 0] : 0) - kbase;
+    const uint64_t cc = (uint64_t)((int64_t)newc[col[e]] - r_lo);
+    keys[e + dl] = ((uint64_t)((int64_t)rr - r_lo) << bits) | cc;
     if (M::HAS_W) vals[e + dl] = (VT)((const typename M::W*)w)[e];
   }
+}
+
+__global__ void k_fill_off(eid* __restrict__ off, int64_t n, int64_t val) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    off[i] = val;
 }
 
 __global__ void k_heads(const uint64_t* __restrict__ k, int64_t n, int32_t* __restrict__ head) {
@@ -392,13 +404,17 @@ __device__ __forceinline__ T shfl_up_t(T v, int d) {
   }
 }
 
+// Group form: rows are relative to the group's first coarse row r_lo and the
+// group's last row is r_hi (off2[r_hi] is written too); coarse entries go to
+// positions U0 + u.
 template <class VT, class OT, bool HAS_V>
 __global__ __launch_bounds__(256) void k_emit(const uint64_t* __restrict__ k,
                                               const VT* __restrict__ v, int64_t n,
                                               const int32_t* __restrict__ head,
                                               const int64_t* __restrict__ uidx, int bits,
                                               vid* __restrict__ col2, OT* __restrict__ w2,
-                                              eid* __restrict__ off2, int64_t C, int64_t U) {
+                                              eid* __restrict__ off2, int64_t r_lo, int64_t r_hi,
+                                              int64_t U0, int64_t U) {
   const uint64_t mask = (bits >= 64) ? ~0ull : ((1ull << bits) - 1ull);
   const int lane = threadIdx.x & 63;
   for (int64_t base = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63); base < n;
@@ -410,16 +426,16 @@ __global__ __launch_bounds__(256) void k_emit(const uint64_t* __restrict__ k,
     int64_t u = ok ? uidx[i + 1] - 1 : -1;
     int64_t glo = 0, ghi = -1, tlo = 0, thi = -1;
     if (ok && head[i]) {
-      col2[u] = (vid)(key & mask);
-      glo = (i == 0) ? 0 : (int64_t)(k[i - 1] >> bits) + 1;
-      ghi = (int64_t)(key >> bits);
+      col2[U0 + u] = (vid)((int64_t)(key & mask) + r_lo);
+      glo = r_lo + ((i == 0) ? 0 : (int64_t)(k[i - 1] >> bits) + 1);
+      ghi = r_lo + (int64_t)(key >> bits);
     }
     if (ok && i == n - 1) {
-      tlo = (int64_t)(key >> bits) + 1;
-      thi = C;
+      tlo = r_lo + (int64_t)(key >> bits) + 1;
+      thi = r_hi;
     }
-    fill_rows(off2, glo, ghi, u);
-    fill_rows(off2, tlo, thi, U);
+    fill_rows(off2, glo, ghi, U0 + u);
+    fill_rows(off2, tlo, thi, U0 + U);
     // segmented inclusive scan over equal u within the wavefront
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -444,79 +460,92 @@ __global__ __launch_bounds__(256) void k_emit(const uint64_t* __restrict__ k,
       const bool started = (u != u_first) || head[base];
       const bool ended = at_end || (lane < 63 ? u_next != u : head[i + 1] != 0);
       if (started && ended) {
-        w2[u] = x;
+        w2[U0 + u] = x;
       } else if constexpr (std::is_same_v<OT, int64_t>) {
-        atomicAdd((unsigned long long*)&w2[u], (unsigned long long)x);
+        atomicAdd((unsigned long long*)&w2[U0 + u], (unsigned long long)x);
       } else {
-        atomicAdd(&w2[u], x);
+        atomicAdd(&w2[U0 + u], x);
       }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-template <class M, class VT, bool HAS_V, class OT>
-static int coarsen_t(int64_t n, const eid* off, const vid* col, const void* w, const vid* c,
-                     const int32_t* gid, const int32_t* keep, const int64_t* delta,
-                     int64_t e_keep, CoarsenBufs& B, eid* off2, vid* col2, void* w2,
-                     int64_t* C_out, int64_t* E2_out, hipStream_t st) {
-  launch_fill_bytes0(B.flag, sizeof(int32_t) * (size_t)n, st);
+// Renumbering of the whole level (louvain-level dense rank of the used ids of
+// the kept graphs): rank[0..n] (rank[n] = C) and newc[v] (-1 in dropped graphs).
+static int coarsen_rank(int64_t n, const vid* c, const int32_t* gid, const int32_t* keep,
+                        CoarsenBufs& B, int64_t* C_out, hipStream_t st) {
+  launch_fill_bytes0(B.flag, sizeof(int32_t) * (size_t)(n > 0 ? n : 1), st);
   k_flag<<<gridc(n, 256, 8192), 256, 0, st>>>(n, c, gid, keep, B.flag);
   scan_exclusive_i32(B.flag, B.rank, n, B.scan_tmp, st);
   int64_t C = 0;
   if (hipMemcpyAsync(&C, B.rank + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
   if (hipStreamSynchronize(st) != hipSuccess) return -1;
   k_newc<<<gridc(n, 256, 8192), 256, 0, st>>>(n, c, gid, keep, B.rank, B.newc);
-  int bits = 1;
-  while ((1ll << bits) < C) ++bits;
-  const int kbits = 2 * bits;
-  int64_t nnz = 0;
-  if (n > 0) {
-    if (hipMemcpyAsync(&nnz, off + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
-    if (hipStreamSynchronize(st) != hipSuccess) return -1;
-  }
-  if (nnz > 0) {
-    const int64_t waves = (nnz + 255) / 256;
-    k_keys<M, VT><<<(unsigned)((waves + 3) / 4), 256, 0, st>>>(n, nnz, off, col, w, B.newc, gid,
-                                                               delta, bits, B.k0, (VT*)B.v0);
-  }
-  VT* vs = nullptr;
-  uint64_t* ks = B.k0;
-  if (e_keep > 0) {
-    const int64_t need = 256 * ((e_keep + RS_TILE - 1) / RS_TILE) * 2 + 2;
-    if (need > B.hist_cap) return -2;
-    ks = radix_sort<VT, HAS_V>(B.k0, B.k1, (VT*)B.v0, (VT*)B.v1, e_keep, kbits, B.hist,
-                               B.scan_tmp, &vs, st);
-  }
-  int64_t U = 0;
-  if (e_keep > 0) {
-    k_heads<<<gridc(e_keep, 256, 8192), 256, 0, st>>>(ks, e_keep, B.head);
-    // uidx[0] = 0, uidx[i+1] = inclusive count
-    scan_exclusive_i32(B.head, B.uidx, e_keep, B.scan_tmp, st);
-    if (hipMemcpyAsync(&U, B.uidx + e_keep, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
-  }
-  if (hipStreamSynchronize(st) != hipSuccess) return -1;
-  // exclusive scan gives uidx[i] = count before i; emit needs the inclusive
-  // count at i, i.e. uidx[i+1]; out[e_keep] = total makes that valid for all i
-  launch_fill_bytes0(w2, sizeof(OT) * (size_t)(U > 0 ? U : 1), st);
-  if (e_keep > 0)
-    k_emit<VT, OT, HAS_V><<<gridc(e_keep, 256, 16384), 256, 0, st>>>(ks, vs, e_keep, B.head,
-                                                                   B.uidx, bits, col2, (OT*)w2,
-                                                                   off2, C, U);
-  else
-    launch_fill_bytes0(off2, sizeof(eid) * (size_t)(C + 1), st);
   *C_out = C;
-  *E2_out = U;
   return 0;
 }
 
-int coarsen_level(int mode_in, int mode_out, int64_t n, const eid* off, const vid* col,
-                  const void* w, const vid* c, const int32_t* gid, const int32_t* keep,
-                  const int64_t* delta, int64_t e_keep, CoarsenBufs& B, eid* off2, vid* col2,
-                  void* w2, int64_t* C_out, int64_t* E2_out, hipStream_t st) {
-#define C_(M_, VT_, HV_, OT_)                                                                \
-  return coarsen_t<M_, VT_, HV_, OT_>(n, off, col, w, c, gid, keep, delta, e_keep, B, off2,  \
-                                      col2, w2, C_out, E2_out, st)
+// One group of consecutive graphs: fine rows [v_lo, v_hi) and entries
+// [e_lo, e_hi) of which e_keep are kept (compacted from kbase on), coarse rows
+// [r_lo, r_hi); the coarse entries are written from U0 on and their number
+// returned in *U_out.
+template <class M, class VT, bool HAS_V, class OT>
+static int coarsen_group_t(int64_t v_lo, int64_t v_hi, int64_t e_lo, int64_t e_hi,
+                           const eid* off, const vid* col, const void* w, const int32_t* gid,
+                           const int64_t* delta, int64_t kbase, int64_t e_keep, int64_t r_lo,
+                           int64_t r_hi, int64_t U0, CoarsenBufs& B, eid* off2, vid* col2,
+                           void* w2, int64_t* U_out, hipStream_t st) {
+  const int64_t Cg = r_hi - r_lo;
+  *U_out = 0;
+  if (Cg <= 0) return 0;
+  if (e_keep <= 0) {
+    // coarse vertices without entries (kept graphs of isolated vertices)
+    k_fill_off<<<gridc(Cg + 1, 256, 4096), 256, 0, st>>>(off2 + r_lo, Cg + 1, U0);
+    return 0;
+  }
+  if (e_keep > B.cap) return -2;
+  int bits = 1;
+  while ((1ll << bits) < Cg) ++bits;
+  const int kbits = 2 * bits;
+  const int64_t waves = (e_hi - e_lo + 255) / 256;
+  k_keys<M, VT><<<(unsigned)((waves + 3) / 4), 256, 0, st>>>(
+      v_lo, v_hi, e_lo, e_hi, off, col, w, B.newc, gid, delta, kbase, r_lo, bits, B.k0,
+      (VT*)B.v0);
+  const int64_t need = 256 * ((e_keep + RS_TILE - 1) / RS_TILE) * 2 + 2;
+  if (need > B.hist_cap) return -2;
+  VT* vs = nullptr;
+  uint64_t* ks = radix_sort<VT, HAS_V>(B.k0, B.k1, (VT*)B.v0, (VT*)B.v1, e_keep, kbits, B.hist,
+                                       B.scan_tmp, &vs, st);
+  k_heads<<<gridc(e_keep, 256, 8192), 256, 0, st>>>(ks, e_keep, B.head);
+  // uidx[0] = 0, uidx[i+1] = inclusive count
+  scan_exclusive_i32(B.head, B.uidx, e_keep, B.scan_tmp, st);
+  int64_t U = 0;
+  if (hipMemcpyAsync(&U, B.uidx + e_keep, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+  if (hipStreamSynchronize(st) != hipSuccess) return -1;
+  // exclusive scan gives uidx[i] = count before i; emit needs the inclusive
+  // count at i, i.e. uidx[i+1]; out[e_keep] = total makes that valid for all i
+  launch_fill_bytes0((OT*)w2 + U0, sizeof(OT) * (size_t)(U > 0 ? U : 1), st);
+  k_emit<VT, OT, HAS_V><<<gridc(e_keep, 256, 16384), 256, 0, st>>>(
+      ks, vs, e_keep, B.head, B.uidx, bits, col2, (OT*)w2, off2, r_lo, r_hi, U0, U);
+  *U_out = U;
+  return 0;
+}
+
+int coarsen_renumber(int64_t n, const vid* c, const int32_t* gid, const int32_t* keep,
+                     CoarsenBufs& B, int64_t* C_out, hipStream_t st) {
+  return coarsen_rank(n, c, gid, keep, B, C_out, st);
+}
+
+int coarsen_group(int mode_in, int mode_out, int64_t v_lo, int64_t v_hi, int64_t e_lo,
+                  int64_t e_hi, const eid* off, const vid* col, const void* w, const int32_t* gid,
+                  const int64_t* delta, int64_t kbase, int64_t e_keep, int64_t r_lo,
+                  int64_t r_hi, int64_t U0, CoarsenBufs& B, eid* off2, vid* col2, void* w2,
+                  int64_t* U_out, hipStream_t st) {
+#define C_(M_, VT_, HV_, OT_)                                                                 \
+  return coarsen_group_t<M_, VT_, HV_, OT_>(v_lo, v_hi, e_lo, e_hi, off, col, w, gid, delta, \
+                                            kbase, e_keep, r_lo, r_hi, U0, B, off2, col2, w2, \
+                                            U_out, st)
   switch (mode_in) {
     case MODE_UNIT:
       if (mode_out == MODE_I32) C_(ModeUnit, int32_t, false, int32_t);
@@ -528,4 +557,26 @@ int coarsen_level(int mode_in, int mode_out, int64_t n, const eid* off, const vi
   }
 #undef C_
   return -1;
+}
+
+int coarsen_level(int mode_in, int mode_out, int64_t n, const eid* off, const vid* col,
+                  const void* w, const vid* c, const int32_t* gid, const int32_t* keep,
+                  const int64_t* delta, int64_t e_keep, CoarsenBufs& B, eid* off2, vid* col2,
+                  void* w2, int64_t* C_out, int64_t* E2_out, hipStream_t st) {
+  int64_t C = 0;
+  int rc = coarsen_rank(n, c, gid, keep, B, &C, st);
+  if (rc) return rc;
+  int64_t nnz = 0;
+  if (n > 0) {
+    if (hipMemcpyAsync(&nnz, off + n, 8, hipMemcpyDeviceToHost, st) != hipSuccess) return -1;
+    if (hipStreamSynchronize(st) != hipSuccess) return -1;
+  }
+  int64_t U = 0;
+  rc = coarsen_group(mode_in, mode_out, 0, n, 0, nnz, off, col, w, gid, delta, 0, e_keep, 0, C,
+                     0, B, off2, col2, w2, &U, st);
+  if (rc) return rc;
+  if (C == 0) launch_fill_bytes0(off2, sizeof(eid), st);
+  *C_out = C;
+  *E2_out = U;
+  return 0;
 }

@@ -21,6 +21,7 @@
 #include "../../include/mgl.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -43,11 +44,17 @@ struct Tuning {
   int hub_agg_per_cu = 4;     // hub-aggregate workgroups sharing a CU's LDS (1..8)
   bool table_check = true;    // false: LDS tables smaller than a bin's rows are accepted
                               // (an overflow then fails the call; used by tests)
+  int64_t coarsen_group = 0;  // > 0: kept entries of one coarsening group (else the default)
+  int hub_stream = 1;         // 1: the hub path runs on a second stream beside the other bins
 };
 
 struct mgl_ctx {
   int device = 0;
   hipStream_t st = nullptr;
+  // second stream of the move pass (the hub path beside the other bins) and
+  // the events that fork it from st and join it back
+  hipStream_t st2 = nullptr;
+  hipEvent_t evfork = nullptr, evjoin = nullptr;
   std::string err;
   int lds_max = 65536;
   double clock_hz = 1.0e8;  // rate of the device clock read by wall_clock64()
@@ -72,6 +79,15 @@ struct mgl_ctx {
     }                                                                                   \
   } while (0)
 
+// Device memory held by the engine's buffers, process-wide (mgl_memory_stats).
+static std::atomic<int64_t> g_dev_live{0}, g_dev_peak{0};
+static void dev_bytes_add(int64_t d) {
+  const int64_t now = g_dev_live.fetch_add(d) + d;
+  int64_t pk = g_dev_peak.load();
+  while (now > pk && !g_dev_peak.compare_exchange_weak(pk, now)) {
+  }
+}
+
 // Device buffer owned by the engine.
 struct DBuf {
   void* p = nullptr;
@@ -81,7 +97,10 @@ struct DBuf {
   DBuf& operator=(const DBuf&) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      dev_bytes_add(-(int64_t)bytes);
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -90,8 +109,12 @@ struct DBuf {
     release();
     if (b == 0) b = 16;
     hipError_t e = hipMalloc(&p, b);
-    if (e == hipSuccess) bytes = b;
-    else p = nullptr;
+    if (e == hipSuccess) {
+      bytes = b;
+      dev_bytes_add((int64_t)b);
+    } else {
+      p = nullptr;
+    }
     return e;
   }
   template <class T> T* as() const { return (T*)p; }
@@ -206,6 +229,11 @@ struct mgl_result {
 };
 
 static const int TRAJ_MAX = 4096;
+// Kept entries one coarsening group of a batch may hold: the sort's keys,
+// values, heads and unique indices take about 44 bytes per entry, so a batch
+// is coarsened in groups of consecutive graphs of at most this many entries
+// (a larger single graph is a group of its own).
+static const int64_t COARSEN_GROUP_ENTRIES = (int64_t)1 << 30;
 // Levels with more vertices than this keep their records in degree-bin order
 // (an 8-byte record array of 2^19 vertices fills 4 MB, one XCD's L2).
 static const int64_t LAYOUT_AUTO_MIN_N = (int64_t)1 << 19;
@@ -324,9 +352,13 @@ struct Engine {
     return MGL_OK;
   }
 
+  int64_t max_graph_nnz0 = 0;  // entries of the largest input graph
+
   mgl_status alloc_coarsen() {
     const int64_t n = n0 > 0 ? n0 : 1;
-    const int64_t e = nnz0 > 0 ? nnz0 : 1;
+    const int64_t grp = tune.coarsen_group > 0 ? tune.coarsen_group : COARSEN_GROUP_ENTRIES;
+    const int64_t e = std::max<int64_t>(
+        1, std::max<int64_t>(max_graph_nnz0, std::min<int64_t>(nnz0, grp)));
     HIPC(ctx, cflag.ensure(4 * n));
     HIPC(ctx, crank.ensure(8 * (n + 1)));
     HIPC(ctx, cnewc.ensure(4 * n));
@@ -353,6 +385,7 @@ struct Engine {
     CB.rowcnt = crowcnt.as<int64_t>();
     CB.scan_tmp = cscan.p;
     CB.hist_cap = hist_cap;
+    CB.cap = e;
     return MGL_OK;
   }
 
@@ -382,9 +415,17 @@ struct Engine {
     HIPC(ctx, hipMemcpyAsync(cursor, cur0.data(), 8 * MGL_MAX_BINS, hipMemcpyHostToDevice, st()));
     HIPC(ctx, hipStreamSynchronize(st()));
     launch_bin_scatter(g.off.as<eid>(), g.n, L, cursor, bperm.as<vid>(), st());
-    // records in degree-bin order once the record array outgrows an L2 slice
+    // records in degree-bin order once a graph's record array outgrows an L2
+    // slice; in a batch of smaller graphs each graph's records stay together
+    // in vertex order (every row's neighbours lie in its own graph)
+    int64_t gmax = g.n;
+    if (g.voff.size() > 2) {
+      gmax = 0;
+      for (size_t i = 0; i + 1 < g.voff.size(); ++i)
+        gmax = std::max<int64_t>(gmax, g.voff[i + 1] - g.voff[i]);
+    }
     rlay = o.record_layout == MGL_LAYOUT_BINNED ||
-           (o.record_layout == MGL_LAYOUT_AUTO && g.n > LAYOUT_AUTO_MIN_N);
+           (o.record_layout == MGL_LAYOUT_AUTO && gmax > LAYOUT_AUTO_MIN_N);
     if (rlay) {
       HIPC(ctx, brpos.ensure(4 * (size_t)n0));
       HIPC(ctx, brinv.ensure(4 * (size_t)n0));
@@ -566,23 +607,38 @@ struct Engine {
     const int64_t first = (rlay && nlist < 0 && L.nb > 1) ? bin_off[1] : 0;
     launch_build_rec(g.mode, r32, vcount(), from, ba.p, brec.p, st(), vlist(),
                      rlay ? brpos.as<vid>() : nullptr, rlay ? brinv.as<vid>() : nullptr, first);
+    // the hub path (several dependent kernels, partly latency bound) runs on
+    // a second stream, forked after the records are built and joined before
+    // anything reads the proposals; the bins write disjoint proposals
+    const int hb = L.nb - 1;
+    const bool fork = tune.hub_stream && ctx->st2 && L.b[hb].shape == SHAPE_GLOBAL &&
+                      bin_n[hb] > 0;
+    if (fork) {
+      HIPC(ctx, hipEventRecord(ctx->evfork, st()));
+      HIPC(ctx, hipStreamWaitEvent(ctx->st2, ctx->evfork, 0));
+    }
     int ne = 0;
     for (int b = L.nb - 1; b >= 1; --b) {
       if (bin_n[b] == 0) continue;
+      hipStream_t sb = (fork && b == hb) ? ctx->st2 : st();
       // per-bin count of aggregated (vertex, neighbour community) pairs
       if (counting) {
         A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS + b;
         A.probes = bcounts.as<unsigned long long>() + 4 * MGL_MAX_BINS;
       }
       A.pos0 = (rlay && nlist < 0 && L.b[b].shape != SHAPE_GLOBAL) ? bin_off[b] : -1;
-      if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b], st()));
+      if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b], sb));
       const int rc = (L.b[b].shape == SHAPE_GLOBAL)
-                         ? launch_hub(g.mode, r32, A, hub, st())
+                         ? launch_hub(g.mode, r32, A, hub, sb)
                          : launch_move_bin(g.mode, r32, L.b[b], A, bperm.as<vid>() + bin_off[b],
-                                           bin_n[b], st());
+                                           bin_n[b], sb);
       if (rc != 0) FAIL(ctx, MGL_UNSUPPORTED, "no move kernel for bin %d (s=%d size=%d)", b, L.b[b].s, L.b[b].size);
-      if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b + 1], st()));
+      if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b + 1], sb));
       ++ne;
+    }
+    if (fork) {
+      HIPC(ctx, hipEventRecord(ctx->evjoin, ctx->st2));
+      HIPC(ctx, hipStreamWaitEvent(st(), ctx->evjoin, 0));
     }
     HIPC(ctx, hipGetLastError());
     return MGL_OK;
@@ -1083,26 +1139,26 @@ mgl_status Engine::level_end(int level) {
   const int nxt = (cur == 1) ? 2 : 1;
   LevelGraph& ng = lg[nxt];
   ng.mode = coarse_mode(g.mode, rec32);
+  // the columns as record positions belong to this level's move kernels; the
+  // next level makes its own (make_bins), so they leave room for the coarsening
+  bcolr.release();
   // upper bounds: C <= n, E2 <= e_keep
   HIPC(ctx, ng.off.ensure(8 * (g.n + 2)));
   HIPC(ctx, ng.col.ensure(4 * std::max<int64_t>(e_keep, 1)));
   HIPC(ctx, ng.w.ensure(std::max<size_t>(wsize(ng.mode), 4) * std::max<int64_t>(e_keep, 1)));
   if (g.mode != MODE_UNIT) {
-    // value ping-pong buffers of the sort, sized by the kept entries
-    HIPC(ctx, cv0.ensure(8 * std::max<int64_t>(e_keep, 1)));
-    HIPC(ctx, cv1.ensure(8 * std::max<int64_t>(e_keep, 1)));
+    // value ping-pong buffers of the sort, sized for one group
+    const int64_t ve = std::max<int64_t>(1, std::min<int64_t>(e_keep, CB.cap));
+    HIPC(ctx, cv0.ensure(8 * ve));
+    HIPC(ctx, cv1.ensure(8 * ve));
     CB.v0 = cv0.p;
     CB.v1 = cv1.p;
   }
+  const int32_t* gidp = g.gid.as<int32_t>();
+  const int64_t* deltap = G > 1 ? bdelta.as<int64_t>() : nullptr;
   int64_t C = 0, E2 = 0;
-  const int rc = coarsen_level(g.mode, ng.mode, g.n, g.off.as<eid>(), g.col.as<vid>(), g.w.p, c,
-                               g.gid.as<int32_t>(), bkeep.as<int32_t>(),
-                               G > 1 ? bdelta.as<int64_t>() : nullptr, e_keep, CB,
-                               ng.off.as<eid>(), ng.col.as<vid>(), ng.w.p, &C, &E2, st());
+  int rc = coarsen_renumber(g.n, c, gidp, bkeep.as<int32_t>(), CB, &C, st());
   if (rc != 0) FAIL(ctx, MGL_DEVICE_ERROR, "coarsening failed (%d)", rc);
-  HIPC(ctx, hipGetLastError());
-  ng.n = C;
-  ng.nnz = E2;
   // per-graph ranges of the coarse union: rank at each graph's first fine vertex
   std::vector<int64_t> idx(G + 1);
   for (int i = 0; i <= G; ++i) idx[i] = g.voff[i];
@@ -1113,6 +1169,32 @@ mgl_status Engine::level_end(int level) {
   ng.voff.resize(G + 1);
   HIPC(ctx, hipMemcpyAsync(ng.voff.data(), dout, 8 * (G + 1), hipMemcpyDeviceToHost, st()));
   HIPC(ctx, hipStreamSynchronize(st()));
+  // groups of consecutive graphs whose kept entries fit the sort's buffers
+  int64_t kbase = 0;
+  for (int i0 = 0; i0 < G;) {
+    int i1 = i0;
+    int64_t ek = 0;
+    while (i1 < G) {
+      const int64_t ke = keep[i1] ? g.eoff[i1 + 1] - g.eoff[i1] : 0;
+      if (i1 > i0 && ek + ke > CB.cap) break;
+      ek += ke;
+      ++i1;
+    }
+    int64_t U = 0;
+    rc = coarsen_group(g.mode, ng.mode, g.voff[i0], g.voff[i1], g.eoff[i0], g.eoff[i1],
+                       g.off.as<eid>(), g.col.as<vid>(), g.w.p, gidp, deltap, kbase, ek,
+                       ng.voff[i0], ng.voff[i1], E2, CB, ng.off.as<eid>(), ng.col.as<vid>(),
+                       ng.w.p, &U, st());
+    if (rc != 0) FAIL(ctx, MGL_DEVICE_ERROR, "coarsening of graphs %d..%d failed (%d)", i0, i1 - 1, rc);
+    E2 += U;
+    kbase += ek;
+    i0 = i1;
+  }
+  HIPC(ctx, hipMemcpyAsync(ng.off.as<eid>() + C, &E2, 8, hipMemcpyHostToDevice, st()));
+  HIPC(ctx, hipStreamSynchronize(st()));
+  HIPC(ctx, hipGetLastError());
+  ng.n = C;
+  ng.nnz = E2;
   // entry ranges: coarse offsets at the graph boundaries
   HIPC(ctx, hipMemcpyAsync(didx, ng.voff.data(), 8 * (G + 1), hipMemcpyHostToDevice, st()));
   k_gather_i64<<<(G + 1 + 255) / 256, 256, 0, st()>>>(ng.off.as<int64_t>(), didx, G + 1, dout);
@@ -1403,6 +1485,8 @@ mgl_status Engine::setup_union(const mgl_csr* graphs, int ng) {
   }
   n0 = vbase0[G];
   nnz0 = eb[G];
+  max_graph_nnz0 = 0;
+  for (int i = 0; i < G; ++i) max_graph_nnz0 = std::max<int64_t>(max_graph_nnz0, graphs[i].nnz);
   if (n0 > 0x7fffffffLL) FAIL(ctx, MGL_UNSUPPORTED, "batch has more than 2^31-1 vertices");
   // per-graph 2m (host_graph.cpp:112-114)
   hgs.assign(G, GState{});
@@ -1551,8 +1635,15 @@ mgl_status mgl_create(mgl_ctx** out, int device) {
   if (device < 0 || device >= ndev) return MGL_INVALID_ARG;
   mgl_ctx* ctx = new mgl_ctx();
   ctx->device = device;
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->st) != hipSuccess) {
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&ctx->st) != hipSuccess ||
+      hipStreamCreate(&ctx->st2) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->evfork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&ctx->evjoin, hipEventDisableTiming) != hipSuccess) {
     (void)hipGetLastError();
+    if (ctx->evjoin) (void)hipEventDestroy(ctx->evjoin);
+    if (ctx->evfork) (void)hipEventDestroy(ctx->evfork);
+    if (ctx->st2) (void)hipStreamDestroy(ctx->st2);
+    if (ctx->st) (void)hipStreamDestroy(ctx->st);
     delete ctx;
     return MGL_DEVICE_ERROR;
   }
@@ -1576,6 +1667,9 @@ mgl_status mgl_create(mgl_ctx** out, int device) {
 void mgl_destroy(mgl_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
+  if (ctx->evjoin) (void)hipEventDestroy(ctx->evjoin);
+  if (ctx->evfork) (void)hipEventDestroy(ctx->evfork);
+  if (ctx->st2) (void)hipStreamDestroy(ctx->st2);
   if (ctx->st) (void)hipStreamDestroy(ctx->st);
   delete ctx;
 }
@@ -1702,6 +1796,13 @@ int mgl_bin_bounds(const mgl_ctx* ctx, int64_t* upper, int max_bins) {
 }
 
 int mgl_device_lds_bytes(const mgl_ctx* ctx) { return ctx ? ctx->lds_max : 0; }
+
+mgl_status mgl_memory_stats(int64_t* live_bytes, int64_t* peak_bytes, int reset_peak) {
+  if (live_bytes) *live_bytes = g_dev_live.load();
+  if (peak_bytes) *peak_bytes = g_dev_peak.load();
+  if (reset_peak) g_dev_peak.store(g_dev_live.load());
+  return MGL_OK;
+}
 
 // Streaming copy of `bytes` between two device buffers with the library's
 // copy kernel (k_stream_copy), `iters` times; returns the best GB/s. Used to
@@ -2190,10 +2291,14 @@ mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64
 // empty lines are skipped; "hub_bucket N", "hub_bucket_unclamped N",
 // "hub_segmented 0|1", "hub_agg_threads 256|512|1024" and "hub_agg_per_cu 1..8"
 // set the hub path; "table_check 0" accepts tables smaller than their bin's rows
-// (a move that overflows one then fails with MGL_DEVICE_ERROR); every other line
+// (a move that overflows one then fails with MGL_DEVICE_ERROR);
+// "hub_stream 0" keeps the hub path on the main stream;
+// "coarsen_group_entries N" bounds the kept entries a batch is coarsened in at
+// once (graphs are grouped up to N, a larger graph alone); every other line
 // is one bin, "upper shape s size", in increasing order of upper (-1:
 // unbounded), whose shape, lane or thread count and table size must name a
-// compiled kernel. Anything else is an error.
+// compiled kernel; a file without bin lines keeps the built-in layout.
+// Anything else is an error.
 static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
   FILE* f = fopen(path, "r");
   if (!f) FAIL(ctx, MGL_INVALID_ARG, "cannot open tuning file %s", path);
@@ -2221,6 +2326,11 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
       tu.hub_segmented = hs != 0;
     } else if (sscanf(p, "table_check %d %c", &hs, &extra) == 1) {
       tu.table_check = hs != 0;
+    } else if (sscanf(p, "hub_stream %d %c", &hs, &extra) == 1) {
+      tu.hub_stream = hs != 0;
+    } else if (sscanf(p, "coarsen_group_entries %lld %c", &hb, &extra) == 1) {
+      if (hb < 1) bad = "coarsen_group_entries";
+      tu.coarsen_group = hb;
     } else if (sscanf(p, "hub_agg_threads %d %c", &hs, &extra) == 1) {
       if (hs != 256 && hs != 512 && hs != 1024) bad = "hub_agg_threads";
       tu.hub_agg_threads = hs;
@@ -2237,6 +2347,7 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
   }
   fclose(f);
   if (bad) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s, line %d: %s", path, lineno, bad);
+  if (L.nb == 0) return MGL_OK;  // no bin lines: the built-in layout
   if (L.nb < 2 || L.b[0].upper != 0 || L.b[0].shape != SHAPE_NONE)
     FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin 0 must be degree 0", path);
   for (int i = 1; i < L.nb; ++i) {

@@ -210,18 +210,19 @@ void scan_exclusive_i32(const int32_t* in, int64_t* out, int64_t n, void* temp, 
 void scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t n, void* temp, hipStream_t st);
 
 struct CoarsenBufs {
-  // all sized by the level-0 graph
+  // per-vertex arrays sized by the level-0 graph, per-entry arrays by cap
   int32_t* flag;      // [n]
   int64_t* rank;      // [n+1]
   vid* newc;          // [n]
-  uint64_t* k0; uint64_t* k1;  // [nnz] keys ping-pong
-  void* v0; void* v1;          // [nnz] values ping-pong (8 B each)
+  uint64_t* k0; uint64_t* k1;  // [cap] keys ping-pong
+  void* v0; void* v1;          // [cap] values ping-pong (8 B each)
   int64_t* hist;      // radix histograms
-  int32_t* head;      // [nnz]
-  int64_t* uidx;      // [nnz+1]
+  int32_t* head;      // [cap]
+  int64_t* uidx;      // [cap+1]
   int64_t* rowcnt;    // [n+1]
   void* scan_tmp;
   int64_t hist_cap;
+  int64_t cap;        // kept entries one group may hold
 };
 // Coarsens the level graph by committed assignment c. Vertices of graphs with
 // keep[g] == 0 are dropped. ebase[g] = first compacted entry of graph g.
@@ -231,3 +232,17 @@ int coarsen_level(int mode_in, int mode_out, int64_t n, const eid* off, const vi
                   const void* w, const vid* c, const int32_t* gid, const int32_t* keep,
                   const int64_t* ebase, int64_t e_keep, CoarsenBufs& B, eid* off2, vid* col2,
                   void* w2, int64_t* C_out, int64_t* E2_out, hipStream_t st);
+// The same in two steps, so that a batch can be coarsened in groups of
+// consecutive graphs whose kept entries fit B.cap (each graph's entries stay in
+// its graph): the renumbering of the whole level (rank[0..n], rank[n] = C,
+// newc), then for each group of fine rows [v_lo, v_hi) and entries [e_lo,
+// e_hi), e_keep of them kept (compacted from kbase on), its coarse rows
+// [r_lo, r_hi) = [rank[v_lo], rank[v_hi]) with entries written from U0 on
+// (*U_out of them). off2[r_hi] is written by the group.
+int coarsen_renumber(int64_t n, const vid* c, const int32_t* gid, const int32_t* keep,
+                     CoarsenBufs& B, int64_t* C_out, hipStream_t st);
+int coarsen_group(int mode_in, int mode_out, int64_t v_lo, int64_t v_hi, int64_t e_lo,
+                  int64_t e_hi, const eid* off, const vid* col, const void* w, const int32_t* gid,
+                  const int64_t* delta, int64_t kbase, int64_t e_keep, int64_t r_lo,
+                  int64_t r_hi, int64_t U0, CoarsenBufs& B, eid* off2, vid* col2, void* w2,
+                  int64_t* U_out, hipStream_t st);

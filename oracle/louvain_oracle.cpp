@@ -31,10 +31,21 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 #include <omp.h>
+
+// ORC_PROGRESS=1 in the environment: one stderr line per inner iteration and
+// per coarsening of the full driver (long runs show where they are).
+static bool progress() {
+  static const int on = [] {
+    const char* e = std::getenv("ORC_PROGRESS");
+    return (e && *e && *e != '0') ? 1 : 0;
+  }();
+  return on != 0;
+}
 
 typedef int64_t i64;
 typedef int32_t i32;
@@ -331,6 +342,9 @@ static void louvain(CSR<W> in, double thr, int max_phases, int max_iter, int ver
       else new_Q = cur_Q;
       L.qtraj.push_back(new_Q);
       ++it;
+      if (progress())
+        fprintf(stderr, "oracle: level %d (%lld vertices) iteration %d Q=%.12f\n", phase,
+                (long long)cur.n, it, new_Q);
     }
     L.iters = it;
     L.q = new_Q;
@@ -341,6 +355,9 @@ static void louvain(CSR<W> in, double thr, int max_phases, int max_iter, int ver
     CSR<W> nxt;
     std::vector<i32> rank;
     coarsen(g, c.data(), nxt, rank);
+    if (progress())
+      fprintf(stderr, "oracle: level %d coarsened to %lld vertices, %lld entries\n", phase,
+              (long long)nxt.n, (long long)nxt.col.size());
     R.graphs.push_back(std::move(nxt));
     ++phase;
   }

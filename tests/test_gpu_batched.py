@@ -125,3 +125,37 @@ def test_batched_workload_256_graphs(ctx, mgl, orc):
         compare(b, oracle_run(orc, g), f"graph {i}")
         if i % 32 == 31:
             print(f"graphs 0..{i} equal to the oracle", flush=True)
+
+
+@pytest.mark.parametrize("group", [1, 40_000, 300_000])
+def test_batched_coarsening_in_groups(ctx, mgl, orc, tmp_path, group):
+    # a batch is coarsened in groups of consecutive graphs whose kept entries
+    # fit a bound (a graph larger than the bound is a group of its own): the
+    # coarse graphs of every level and the runs equal the oracle's whatever
+    # the grouping (group = 1: every graph alone)
+    tf = tmp_path / "group.txt"
+    tf.write_text(f"coarsen_group_entries {group}\n")
+    graphs = []
+    for i in range(16):
+        if i % 2 == 0:
+            graphs.append(mgl.gen_sbm(800 + 500 * i, 4 + i, min_size=8, deg_in=16, deg_out=3,
+                                      seed=700 + i, weights=1))
+        else:
+            graphs.append(mgl.gen_rmat(10 + (i % 4), 8 + i, seed=800 + i,
+                                       weights=1 if i % 3 else 0, n_limit=700 + 400 * i))
+    graphs.append(mgl.CSR(np.zeros(41, np.int64), np.zeros(0, np.int32)))  # isolated only
+    print("entries", [g.nnz for g in graphs], flush=True)
+    batch = ctx.louvain_batched(graphs, keep_levels=True, keep_graphs=True,
+                                tuning_file=str(tf))
+    for i, (g, b) in enumerate(zip(graphs, batch)):
+        w = None if g.w is None else g.w.astype(np.float64)
+        ref = orc.louvain(g.offsets, g.cols, w, mode=0)
+        compare(b, ref, f"graph {i}")
+        # batch weights carry the batch's scale; the levels' graphs are in graph units
+        S = 0 if w is None else orc.scale_of(w)
+        for l in range(1, len(b.levels)):
+            go, gc, gw = ref.graphs[l]
+            np.testing.assert_array_equal(b.levels[l].graph.offsets, go, err_msg=f"{i}/{l}")
+            np.testing.assert_array_equal(b.levels[l].graph.cols, gc, err_msg=f"{i}/{l}")
+            np.testing.assert_array_equal(b.levels[l].graph_w64, np.ldexp(gw, -S),
+                                          err_msg=f"{i}/{l}")

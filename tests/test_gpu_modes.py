@@ -63,8 +63,10 @@ def test_fp32_hash_vs_fp64_lognormal(ctx, mgl, orc):
           f"(recomputed {q64:.9f}) |dQ|={abs(q32 - q64):.3e} agreement={agr:.4f}", flush=True)
     assert abs(r32.final_q - q32) <= 1e-6 * abs(q32)
     assert abs(r64.final_q - q64) <= 1e-6 * abs(q64)
-    assert abs(q32 - q64) < 2e-3
-    assert agr > 0.9
+    # observed on the MI355X at this size: |dQ| = 0 and agreement 1.0 (FP32 hash
+    # sums of lognormal weights are order dependent, so a small margin)
+    assert abs(q32 - q64) <= 1e-6
+    assert agr >= 0.995
 
 
 @pytest.mark.timeout(600)
