@@ -208,6 +208,19 @@ mgl_status mgl_session_modularity(mgl_session* s, double* q);
 mgl_status mgl_modularity_dev(mgl_ctx* ctx, int64_t n, int64_t nnz, const int64_t* d_offsets,
                               const int32_t* d_cols, const float* d_w, const int32_t* d_assign,
                               double* q);
+/* The same as mgl_modularity_dev without any device allocation. No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc). Its
+ * working memory is a fixed array in device memory, one per device, held for
+ * the life of the process (64 MiB + 256 B): graphs of at most
+ * MGL_NOALLOC_MAX_VERTICES vertices, larger ones are refused with
+ * MGL_INVALID_ARG before anything runs. The array is cleared at the start of
+ * each call; calls that overlap on one device (from any context) race on it
+ * and must be serialised by the caller. */
+#define MGL_NOALLOC_MAX_VERTICES (1 << 23)
+mgl_status mgl_modularity_dev_noalloc(mgl_ctx* ctx, int64_t n, int64_t nnz,
+                                      const int64_t* d_offsets, const int32_t* d_cols,
+                                      const float* d_w, const int32_t* d_assign, double* q);
 /* Community totals a[n] (as doubles, in graph units) of the committed state. */
 mgl_status mgl_session_get_weights(mgl_session* s, double* a, double* k);
 /* Replaces compute_community_weight(_local_atomic) (louvain.cu:1299-1396,

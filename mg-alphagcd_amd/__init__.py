@@ -145,6 +145,8 @@ _sig("mgl_device_lds_bytes", _i, [_vp])
 _sig("mgl_membench", _i, [_vp, _i64, _i, C.POINTER(_d)])
 _sig("mgl_memory_stats", _i, [C.POINTER(_i64), C.POINTER(_i64), _i])
 _sig("mgl_modularity_dev", _i, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, C.POINTER(_d)])
+_sig("mgl_modularity_dev_noalloc", _i, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, C.POINTER(_d)])
+NOALLOC_MAX_VERTICES = 1 << 23
 
 
 def _p(a, t):
@@ -431,6 +433,15 @@ class Context:
         q = C.c_double()
         self._err(_lib.mgl_modularity_dev(self._h, n, nnz, d_offsets, d_cols, d_w or None,
                                           d_assign, C.byref(q)))
+        return q.value
+
+    def modularity_dev_noalloc(self, n: int, nnz: int, d_offsets: int, d_cols: int, d_w: int,
+                               d_assign: int) -> float:
+        """modularity_dev without any device allocation (graphs of at most
+        NOALLOC_MAX_VERTICES vertices; overlapping calls on one device race)."""
+        q = C.c_double()
+        self._err(_lib.mgl_modularity_dev_noalloc(self._h, n, nnz, d_offsets, d_cols,
+                                                  d_w or None, d_assign, C.byref(q)))
         return q.value
 
     def autotune(self, g: CSR, path: str):

@@ -1623,6 +1623,9 @@ void mgl_opts_default(mgl_opts* o) {
 int mgl_version_major(void) { return 1; }
 
 static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu);
+}  // extern "C"
+int mgl_noalloc_setup(int device);  // mgl_modularity.hip
+extern "C" {
 
 mgl_status mgl_create(mgl_ctx** out, int device) {
   if (!out) return MGL_INVALID_ARG;
@@ -1652,6 +1655,8 @@ mgl_status mgl_create(mgl_ctx** out, int device) {
           hipSuccess &&
       lds > 0)
     ctx->lds_max = lds;
+  // the allocation-free entry point's working memory on this device
+  (void)mgl_noalloc_setup(device);
   int khz = 0;
   if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device) == hipSuccess &&
       khz > 0)

@@ -228,3 +228,78 @@ extern "C" mgl_status mgl_modularity_dev(mgl_ctx* ctx, int64_t n, int64_t nnz,
   if (s) mgl_set_error(ctx, msg);
   return s;
 }
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the same kernels with their working memory in a
+// file-scope device array. Each device the library runs on holds its own copy
+// for the life of the process; its address on a device comes from
+// hipGetSymbolAddress, taken once per device with that device current (at
+// mgl_create). Host code never passes the array's name itself to a kernel.
+static __device__ __align__(256) unsigned char
+    scratch[MOD_HEADER_BYTES + 8 * (size_t)MGL_NOALLOC_MAX_VERTICES];
+static_assert(MGL_NOALLOC_MAX_VERTICES <= 0x7fffffff, "vertex limit fits int32 ids");
+
+#define MOD_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[MOD_MAX_DEVICES];  // device address of scratch per device
+static int g_scratch_state[MOD_MAX_DEVICES];  // 0 not looked up, 1 ok, -1 failed
+
+// Looks up the array's address on `device`, which must be the current device.
+int mgl_noalloc_setup(int device) {
+  if (device < 0 || device >= MOD_MAX_DEVICES) return -1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_state[device] != 0) return g_scratch_state[device] > 0 ? 0 : -1;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    g_scratch_state[device] = -1;
+    return -1;
+  }
+  g_scratch_ptr[device] = p;
+  g_scratch_state[device] = 1;
+  return 0;
+}
+
+extern "C" mgl_status mgl_modularity_dev_noalloc(mgl_ctx* ctx, int64_t n, int64_t nnz,
+                                                 const int64_t* d_offsets,
+                                                 const int32_t* d_cols, const float* d_w,
+                                                 const int32_t* d_assign, double* q) {
+  if (!ctx) return MGL_INVALID_ARG;
+  std::string msg;
+  mgl_status s = mod_args(ctx, n, nnz, d_offsets, d_cols, d_assign, q, msg);
+  if (s) { mgl_set_error(ctx, msg); return s; }
+  if (n > MGL_NOALLOC_MAX_VERTICES) {
+    char b[160];
+    snprintf(b, sizeof(b), "%lld vertices exceed the %d the allocation-free entry point holds",
+             (long long)n, (int)MGL_NOALLOC_MAX_VERTICES);
+    mgl_set_error(ctx, b);
+    return MGL_INVALID_ARG;
+  }
+  const int dev = mgl_ctx_device(ctx);
+  if (hipSetDevice(dev) != hipSuccess) {
+    (void)hipGetLastError();
+    mgl_set_error(ctx, "hipSetDevice failed");
+    return MGL_DEVICE_ERROR;
+  }
+  if (mgl_noalloc_setup(dev) != 0) {
+    mgl_set_error(ctx, "the allocation-free working memory has no address on this device");
+    return MGL_DEVICE_ERROR;
+  }
+  unsigned char* scr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    scr = (unsigned char*)g_scratch_ptr[dev];
+  }
+  int err = 0;
+  const hipError_t e =
+      mod_run(n, nnz, d_offsets, d_cols, d_w, d_assign, scr, mgl_ctx_stream(ctx), q, &err);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    mgl_set_error(ctx, std::string("modularity: ") + hipGetErrorString(e));
+    return MGL_DEVICE_ERROR;
+  }
+  s = mod_result(err, msg);
+  if (s) mgl_set_error(ctx, msg);
+  return s;
+}
