@@ -596,8 +596,18 @@ struct Engine {
     return A;
   }
 
+  // MoveArgs of the hub bin (its counters, no record positions by bin).
+  MoveArgs hub_args(MoveArgs A, int b) {
+    if (counting) {
+      A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS + b;
+      A.probes = bcounts.as<unsigned long long>() + 4 * MGL_MAX_BINS;
+    }
+    A.pos0 = -1;
+    return A;
+  }
+
   // All move kernels of the level (calculate_eicj_and_move_vertex_bin,
-  // louvain.cu:1032-1296), one stream, largest bins first.
+  // louvain.cu:1032-1296), largest bins first.
   mgl_status move(const vid* from, vid* to, int up, void* vin, bool time_bins) {
     LevelGraph& g = LG();
     MoveArgs A = move_args(from, to, up, vin);
@@ -607,20 +617,32 @@ struct Engine {
     const int64_t first = (rlay && nlist < 0 && L.nb > 1) ? bin_off[1] : 0;
     launch_build_rec(g.mode, r32, vcount(), from, ba.p, brec.p, st(), vlist(),
                      rlay ? brpos.as<vid>() : nullptr, rlay ? brinv.as<vid>() : nullptr, first);
-    // the hub path (several dependent kernels, partly latency bound) runs on
-    // a second stream, forked after the records are built and joined before
-    // anything reads the proposals; the bins write disjoint proposals
+    // the hub path (several dependent kernels, partly latency bound) can run
+    // on a second stream beside the bins of short rows (degree <= 128: many
+    // small, latency-bound launches): it is forked once the larger bins are
+    // done and joined before anything reads the proposals; the bins write
+    // disjoint proposals and only read the records
     const int hb = L.nb - 1;
     const bool fork = tune.hub_stream && ctx->st2 && L.b[hb].shape == SHAPE_GLOBAL &&
                       bin_n[hb] > 0;
-    if (fork) {
-      HIPC(ctx, hipEventRecord(ctx->evfork, st()));
-      HIPC(ctx, hipStreamWaitEvent(ctx->st2, ctx->evfork, 0));
-    }
+    int fork_at = 0;  // the hub path starts when the bins above this one are done
+    if (fork)
+      for (int b = hb - 1; b >= 1; --b)
+        if (L.b[b].upper <= 128) { fork_at = b; break; }
     int ne = 0;
     for (int b = L.nb - 1; b >= 1; --b) {
+      if (fork && b == fork_at) {
+        HIPC(ctx, hipEventRecord(ctx->evfork, st()));
+        HIPC(ctx, hipStreamWaitEvent(ctx->st2, ctx->evfork, 0));
+        if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * hb], ctx->st2));
+        if (launch_hub(g.mode, r32, hub_args(A, hb), hub, ctx->st2) != 0)
+          FAIL(ctx, MGL_UNSUPPORTED, "no hub-path kernels for mode %d", g.mode);
+        if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * hb + 1], ctx->st2));
+        HIPC(ctx, hipEventRecord(ctx->evjoin, ctx->st2));
+      }
       if (bin_n[b] == 0) continue;
-      hipStream_t sb = (fork && b == hb) ? ctx->st2 : st();
+      if (fork && b == hb) continue;  // launched beside the short-row bins
+      hipStream_t sb = st();
       // per-bin count of aggregated (vertex, neighbour community) pairs
       if (counting) {
         A.stats = bcounts.as<unsigned long long>() + 2 * MGL_MAX_BINS + b;
@@ -636,10 +658,17 @@ struct Engine {
       if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * b + 1], sb));
       ++ne;
     }
-    if (fork) {
+    if (fork && fork_at == 0) {
+      // no short-row bin: the hub path runs after the others, on its stream
+      HIPC(ctx, hipEventRecord(ctx->evfork, st()));
+      HIPC(ctx, hipStreamWaitEvent(ctx->st2, ctx->evfork, 0));
+      if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * hb], ctx->st2));
+      if (launch_hub(g.mode, r32, hub_args(A, hb), hub, ctx->st2) != 0)
+        FAIL(ctx, MGL_UNSUPPORTED, "no hub-path kernels for mode %d", g.mode);
+      if (time_bins) HIPC(ctx, hipEventRecord(ev[2 * hb + 1], ctx->st2));
       HIPC(ctx, hipEventRecord(ctx->evjoin, ctx->st2));
-      HIPC(ctx, hipStreamWaitEvent(st(), ctx->evjoin, 0));
     }
+    if (fork) HIPC(ctx, hipStreamWaitEvent(st(), ctx->evjoin, 0));
     HIPC(ctx, hipGetLastError());
     return MGL_OK;
   }
