@@ -263,7 +263,8 @@ def timed_passes(sess, steps, warmup):
             f"({s['dominant_vertices']} v, {s['dominant_entries']} e) | hub path "
             f"{s['hub_ms']:.3f} ms ({s['hub_vertices']} v, {s['hub_entries']} e) | moved "
             f"{s['moved']} distinct {s['distinct']} Q={s['q']:.9f}")
-        log("   bin ms: " + " ".join(f"{x:.3f}" for x in s["bin_ms"]))
+        log("   bin ms: " + " ".join(f"{x:.3f}" for x in s["bin_ms"]) +
+            (f" (bins 1-{s['hub_beside']} beside the hub path)" if s["hub_beside"] else ""))
     return stats, wall
 
 

@@ -154,7 +154,8 @@ typedef struct {
   double update_ms;    /* last pass: community-weight update */
   double modularity_ms;/* last pass: sum_in / sum a^2 reduction (+ E-pass when not fused) */
   double pass_ms;      /* last pass: whole inner-loop body incl. commit */
-  double dominant_ms;  /* last pass: duration of the longest single-kernel move launch */
+  double dominant_ms;  /* last pass: duration of the longest single-kernel move launch
+                          among those that ran alone (not beside the hub path) */
   int dominant_bin;    /* bin id of that launch */
   int64_t dominant_vertices, dominant_entries; /* vertices / CSR entries that launch processed */
   int64_t moved;       /* vertices whose proposal differs from their community */
@@ -170,6 +171,9 @@ typedef struct {
   /* last pass: duration of each bin's move launch (the hub bin: its whole path), 0 if empty */
   int n_bins;
   double bin_ms[16];
+  /* last pass: bins 1..hub_beside ran beside the hub path on the second stream
+   * (their bin_ms span includes the time they shared the device), 0 if none */
+  int hub_beside;
 } mgl_pass_stats;
 
 mgl_status mgl_session_create(mgl_ctx* ctx, const mgl_csr* g, const mgl_opts* o,

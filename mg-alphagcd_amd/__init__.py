@@ -77,7 +77,7 @@ class PassStats(C.Structure):
                 ("hub_vertices", C.c_int64), ("hub_entries", C.c_int64),
                 ("hub_distinct", C.c_int64), ("lds_inserts", C.c_int64),
                 ("lds_probes", C.c_int64), ("lds_multi_probe", C.c_int64),
-                ("n_bins", C.c_int), ("bin_ms", C.c_double * 16)]
+                ("n_bins", C.c_int), ("bin_ms", C.c_double * 16), ("hub_beside", C.c_int)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -369,7 +369,7 @@ def make_opts(version=2, threshold=1e-6, max_phases=1000, max_iter=0, precision=
 
 
 class Context:
-    """One device and one stream (mgl_create)."""
+    """One device and its streams (mgl_create)."""
 
     def __init__(self, device: int = 0):
         h = C.c_void_p()

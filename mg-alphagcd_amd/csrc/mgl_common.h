@@ -198,6 +198,14 @@ __device__ __forceinline__ T block_sum(T v, T* red /* [BS/64] */) {
   return s;
 }
 
+// Workgroup barrier that orders LDS accesses only: loads from global memory
+// issued before it stay in flight (__syncthreads also waits for those).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int BS>
 __device__ __forceinline__ void block_argmax(double& bg, vid& bd, bool small, double* rg,
                                              vid* rd) {

@@ -46,6 +46,8 @@ struct Tuning {
                               // (an overflow then fails the call; used by tests)
   int64_t coarsen_group = 0;  // > 0: kept entries of one coarsening group (else the default)
   int hub_stream = 1;         // 1: the hub path runs on a second stream beside the other bins
+  int64_t hub_fork_upper = 512; // the hub path starts once the bins of rows longer than this
+                                // are launched (0: after every bin)
 };
 
 struct mgl_ctx {
@@ -284,6 +286,7 @@ struct Engine {
   mgl_pass_stats stats{};
   bool timing = false;    // HIP events around each bin and stage
   bool counting = false;  // per-bin distinct pairs and LDS probe counters (global atomics)
+  int shared_from = 0;    // last move: bins 1..shared_from ran beside the hub path (0: none)
 
   Tuning tune;  // this run's bin layout and hub settings
 
@@ -628,7 +631,8 @@ struct Engine {
     int fork_at = 0;  // the hub path starts when the bins above this one are done
     if (fork)
       for (int b = hb - 1; b >= 1; --b)
-        if (L.b[b].upper <= 128) { fork_at = b; break; }
+        if (L.b[b].upper <= tune.hub_fork_upper) { fork_at = b; break; }
+    shared_from = fork ? fork_at : 0;
     int ne = 0;
     for (int b = L.nb - 1; b >= 1; --b) {
       if (fork && b == fork_at) {
@@ -2069,7 +2073,9 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   // counters of this pass's move, if mgl_session_count ran on the same state
   unsigned long long dh[MGL_MAX_BINS + 3] = {};
   if (s->count_it == s->it) memcpy(dh, s->cnt, sizeof(dh));
-  // the dominant launch among bins served by one kernel; the hub path apart
+  // the dominant launch among bins served by one kernel that ran alone on the
+  // device (a bin beside the hub path shares the CUs, so its span is not its
+  // own time); the hub path apart
   int hub_bin = -1;
   st.n_bins = E.L.nb;
   for (int b = E.L.nb - 1; b >= 1; --b) {
@@ -2081,6 +2087,7 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
       hub_bin = b;
       continue;
     }
+    if (b <= E.shared_from) continue;
     if (ms > st.dominant_ms) {
       st.dominant_ms = ms;
       st.dominant_bin = b;
@@ -2106,6 +2113,7 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   st.lds_multi_probe = (int64_t)dh[MGL_MAX_BINS + 2];
   st.moved = gs.moved;
   st.q = gs.q_cur;
+  st.hub_beside = hub_bin >= 0 ? E.shared_from : 0;
   // a rejected proposal ends the level; the next pass restarts the loop (a
   // caller that times passes can call mgl_session_reset in between instead)
   s->restart = !gs.active;
@@ -2326,7 +2334,9 @@ mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64
 // "hub_segmented 0|1", "hub_agg_threads 256|512|1024" and "hub_agg_per_cu 1..8"
 // set the hub path; "table_check 0" accepts tables smaller than their bin's rows
 // (a move that overflows one then fails with MGL_DEVICE_ERROR);
-// "hub_stream 0" keeps the hub path on the main stream;
+// "hub_stream 0" keeps the hub path on the main stream; "hub_fork_upper N"
+// starts it once the bins of rows longer than N entries are launched (default
+// 512, 0 = after every bin);
 // "coarsen_group_entries N" bounds the kept entries a batch is coarsened in at
 // once (graphs are grouped up to N, a larger graph alone); every other line
 // is one bin, "upper shape s size", in increasing order of upper (-1:
@@ -2362,6 +2372,9 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
       tu.table_check = hs != 0;
     } else if (sscanf(p, "hub_stream %d %c", &hs, &extra) == 1) {
       tu.hub_stream = hs != 0;
+    } else if (sscanf(p, "hub_fork_upper %lld %c", &hb, &extra) == 1) {
+      if (hb < 0) bad = "hub_fork_upper";
+      tu.hub_fork_upper = hb;
     } else if (sscanf(p, "coarsen_group_entries %lld %c", &hb, &extra) == 1) {
       if (hb < 1) bad = "coarsen_group_entries";
       tu.coarsen_group = hb;

@@ -424,7 +424,6 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
       TB tb;
       tb.init(smem, Ht);
       for (uint32_t s = threadIdx.x; s < Ht; s += BS) tb.clear(s);
-      __syncthreads();
       int own[IT];
       AT oa[IT];
       vid key[IT];
@@ -440,6 +439,7 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
           wv[j] = M::HAS_W ? ((const ST*)src.w)[q] : (ST)1;
         }
       }
+      lds_barrier();  // the table is cleared; the candidate loads stay in flight
 #pragma unroll
       for (int j = 0; j < IT; ++j)
         own[j] = key[j] != MGL_EMPTY ? tb.ins(key[j], wv[j], Ht, false, 0, pc) : -1;

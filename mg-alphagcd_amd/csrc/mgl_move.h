@@ -249,18 +249,14 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
 }
 
 // ---------------------------------------------------------------------------
-// Gathers and inserts the (at most IT per lane) entries of v's row.
-template <class M, class AT, int IT, class TB>
-__device__ __forceinline__ void own_row(const MoveArgs& A, vid vp, vid ci, eid e0, eid e1,
-                                        int lane, int stride, TB& tb, uint32_t H, bool pow2,
-                                        int lg, int (&own)[IT], AT (&oa)[IT],
-                                        typename M::S& eici, typename M::S& selfw,
-                                        ProbeCount& pc) {
-  typedef typename M::S ST;
+// Issues the column loads and record gathers of the (at most IT per lane)
+// entries of a row; own_insert consumes them. In between, the kernels wait at
+// an LDS-only barrier for their table to be cleared, so the gathers are in
+// flight while the workgroup synchronises.
+template <class AT, int IT>
+__device__ __forceinline__ void own_gather(const MoveArgs& A, eid e0, eid e1, int lane,
+                                           int stride, vid (&u)[IT], Rec<AT> (&r)[IT]) {
   const Rec<AT>* R = (const Rec<AT>*)A.rec;
-  const bool v1 = A.version == 1;
-  vid u[IT];
-  Rec<AT> r[IT];
 #pragma unroll
   for (int j = 0; j < IT; ++j) {
     const eid e = e0 + lane + (eid)j * stride;
@@ -269,6 +265,19 @@ __device__ __forceinline__ void own_row(const MoveArgs& A, vid vp, vid ci, eid e
 #pragma unroll
   for (int j = 0; j < IT; ++j)
     if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
+}
+
+// Inserts the gathered entries: candidates into the table (own[j] = the slot
+// if this lane created the key), the own community's entries into eici and
+// the self-loop.
+template <class M, class AT, int IT, class TB>
+__device__ __forceinline__ void own_insert(const MoveArgs& A, vid vp, vid ci, eid e0, int lane,
+                                           int stride, TB& tb, uint32_t H, bool pow2, int lg,
+                                           const vid (&u)[IT], const Rec<AT> (&r)[IT],
+                                           int (&own)[IT], AT (&oa)[IT], typename M::S& eici,
+                                           typename M::S& selfw, ProbeCount& pc) {
+  typedef typename M::S ST;
+  const bool v1 = A.version == 1;
 #pragma unroll
   for (int j = 0; j < IT; ++j) {
     own[j] = -1;
@@ -308,23 +317,30 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
   int g = 0;
   bool ok = gi < nb;
   if (ok) { v = perm[gi]; ok = vertex_active<M>(A, v, g); }
-  __syncthreads();
   ST eici = 0, selfw = 0;
-  vid ci = 0;
+  vid ci = 0, vp = 0;
   KT acik = 0;
+  eid e0 = 0, e1 = 0;
   int own[IT];
   AT oa[IT];
+  vid u[IT];
+  Rec<AT> r[IT];
   ProbeCount pc;
 #pragma unroll
   for (int j = 0; j < IT; ++j) own[j] = -1;
   if (ok) {
-    const vid vp = rec_pos(A, v, gi);
+    vp = rec_pos(A, v, gi);
     const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
     ci = rv.c;
     acik = (KT)rv.a;
-    own_row<M, AT, IT, TB>(A, vp, ci, A.off[v], A.off[v + 1], lane, S, tb, H, true, LG, own, oa,
-                           eici, selfw, pc);
+    e0 = A.off[v];
+    e1 = A.off[v + 1];
   }
+  own_gather<AT, IT>(A, e0, e1, lane, S, u, r);  // no entries unless ok
+  lds_barrier();  // the tables are cleared
+  if (ok)
+    own_insert<M, AT, IT, TB>(A, vp, ci, e0, lane, S, tb, H, true, LG, u, r, own, oa, eici,
+                              selfw, pc);
   __syncthreads();
   if (ok) {
     eici = group_sum<S>(eici);
@@ -378,13 +394,21 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   const vid vp = rec_pos(A, v, gi);
   const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
   const vid ci = rv.c;
-  __syncthreads();
+  const eid e0 = A.off[v], e1 = A.off[v + 1];
   ST eici = 0, selfw = 0;
   int own[IT];
   AT oa[IT];
+  vid u[IT];
+  Rec<AT> r[IT];
   ProbeCount pc;
-  own_row<M, AT, IT, TB>(A, vp, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, tb, H, false, 0, own,
-                         oa, eici, selfw, pc);
+  // the table must be cleared before any insert: workgroups of up to 512
+  // threads wait at an LDS-only barrier after issuing their gathers; a
+  // 1024-thread workgroup (one per CU) waits before them, which measured faster
+  if constexpr (BS >= 1024) __syncthreads();
+  own_gather<AT, IT>(A, e0, e1, threadIdx.x, BS, u, r);
+  if constexpr (BS < 1024) lds_barrier();
+  own_insert<M, AT, IT, TB>(A, vp, ci, e0, threadIdx.x, BS, tb, H, false, 0, u, r, own, oa, eici,
+                            selfw, pc);
   if constexpr (CNT) flush_probes(A.probes, pc);
   flag_overflow(A.err, pc);
   __syncthreads();

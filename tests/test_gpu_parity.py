@@ -288,6 +288,7 @@ _GOOD_BINS = "0 0 0 0\n32 1 32 6\n-1 3 0 0\n"
     ("hub_agg_threads 300\n" + _GOOD_BINS, "hub_agg_threads"),
     ("hub_agg_per_cu 9\n" + _GOOD_BINS, "hub_agg_per_cu"),
     ("hub_bucket 0\n" + _GOOD_BINS, "hub_bucket"),
+    ("hub_fork_upper -1\n" + _GOOD_BINS, "hub_fork_upper"),
     ("hub_buckets 3072\n" + _GOOD_BINS, "unknown line"),
     ("hub_bucket 3072 4096\n" + _GOOD_BINS, "unknown line"),
     ("0 0 0 0\n32 1 32 6 7\n-1 3 0 0\n", "unknown line"),
@@ -343,6 +344,28 @@ def test_hub_graph_full_parity(ctx, mgl, orc, form, version):
         assert res.precision == mgl.PREC_FP64
         compare_fp64(res, orc.louvain(g.offsets, g.cols, g.w.astype(np.float64), mode=1,
                                       version=version))
+
+
+@pytest.mark.parametrize("tuning", ["hub_stream 0", "hub_fork_upper 0", "hub_fork_upper 128",
+                                    "hub_fork_upper 2048", "hub_fork_upper 1000000"])
+def test_hub_stream_placement_parity(ctx, mgl, orc, tmp_path, tuning):
+    # the hub path on the main stream, or forked onto the second stream after
+    # all bins, before the short-row bins, before rows of up to 2,048 entries,
+    # or before every bin: the same results as the oracle in each case
+    tf = tmp_path / "tune.txt"
+    tf.write_text(tuning + "\n")
+    g = hub_graph(mgl)
+    res = ctx.louvain(g, tuning_file=str(tf), keep_levels=True, keep_graphs=True)
+    compare(res, orc.louvain(g.offsets, g.cols, None))
+    s = ctx.session(g, tuning_file=str(tf))
+    k = orc.vertex_weights(g.offsets)
+    c = (np.random.default_rng(5).integers(0, g.n, g.n) // 7 * 7).astype(np.int32)
+    s.set_assign(c)
+    a = orc.community_totals(k, c, g.n)
+    for up in (True, False):
+        np.testing.assert_array_equal(
+            s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
+    s.close()
 
 
 def test_edge_cases(ctx, mgl, orc):
