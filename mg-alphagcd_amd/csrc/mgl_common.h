@@ -198,6 +198,28 @@ __device__ __forceinline__ T block_sum(T v, T* red /* [BS/64] */) {
   return s;
 }
 
+// Sums of a and b over a workgroup with one barrier; the caller has passed a
+// barrier since any earlier use of ra and rb.
+template <int BS, class T>
+__device__ __forceinline__ void block_sum2(T& a, T& b, T* ra, T* rb /* [BS/64] each */) {
+  a = group_sum<64>(a);
+  b = group_sum<64>(b);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) {
+    ra[w] = a;
+    rb[w] = b;
+  }
+  __syncthreads();
+  T sa = 0, sb = 0;
+#pragma unroll
+  for (int i = 0; i < BS / 64; ++i) {
+    sa += ra[i];
+    sb += rb[i];
+  }
+  a = sa;
+  b = sb;
+}
+
 // Workgroup barrier that orders LDS accesses only: loads from global memory
 // issued before it stay in flight (__syncthreads also waits for those).
 __device__ __forceinline__ void lds_barrier() {

@@ -42,6 +42,8 @@ struct Tuning {
   int hub_segmented = 1;      // 0: the hub path always scatters into buckets
   int hub_agg_threads = 512;  // threads per hub-aggregate workgroup (256, 512, 1024)
   int hub_agg_per_cu = 4;     // hub-aggregate workgroups sharing a CU's LDS (1..8)
+  int hub_count_threads = 256; // threads per hub-count workgroup (256, 512, 1024) of the
+                               // segmented form; every workgroup takes 8,192 entries
   bool table_check = true;    // false: LDS tables smaller than a bin's rows are accepted
                               // (an overflow then fails the call; used by tests)
   int64_t coarsen_group = 0;  // > 0: kept entries of one coarsening group (else the default)
@@ -177,6 +179,12 @@ static BinLayout layout_for(const mgl_ctx* ctx, const Tuning& tu, int mode, bool
   BinLayout L = tu.have_bins ? tu.bins : default_bins(ctx->lds_max, sb);
   for (int i = 0; i < L.nb; ++i) {
     BinSpec& b = L.b[i];
+    if (b.shape == SHAPE_PHASE) {
+      // any table size works (a full phase is split); it must fit the LDS
+      const int64_t cap = (int64_t)(ctx->lds_max - 2048) / sb;
+      if (b.size > cap) b.size = (int)cap;
+      b.size = prime_at_most(b.size);
+    }
     if (b.shape == SHAPE_BLOCK) {
       const bool owner = (b.upper + b.s - 1) / b.s <= 16;
       const int64_t cap =
@@ -268,7 +276,7 @@ struct Engine {
   std::vector<int64_t> bin_n, bin_off;
   // hub plan of the current level (last bin)
   DBuf hchunk, hv, hbk, hbkhub, htmpc, htmpw, hbktc, hbktw, hbcount, hboff, hbcur, heici, hself,
-      hbgain, hbdst, hgk, hgv, hscan, htmpa, hbkta, hga, hchunk0, hcbo0, hcbo, hgalloc, hbmeta;
+      hbgain, hbdst, hgk, hgv, hscan, htmpa, hbkta, hga, hcbo, hgalloc, hbmeta, hgin;
   HubPlan hub;
   // coarsening buffers
   DBuf cflag, crank, cnewc, ck0, ck1, cv0, cv1, chist, chead, cuidx, crowcnt, cscan;
@@ -468,43 +476,51 @@ struct Engine {
         BK = std::min<int64_t>(BK, (int64_t)hub_agg_cap() * 4 / 5);
         if (BK < 64) BK = 64;
       }
-      std::vector<int64_t> chunk, hbkv(m + 1, 0), bkhub, chunk0(m + 1, 0), cbo0(m, 0);
+      std::vector<int32_t> hgid(m, 0);
+      if (g.gid.p) {
+        // graph of each hub vertex (batches)
+        DBuf hg;
+        HIPC(ctx, hg.ensure(4 * m));
+        launch_gather_i32(g.gid.as<int32_t>(), hperm, m, hg.as<int32_t>(), st());
+        HIPC(ctx, hipMemcpyAsync(hgid.data(), hg.p, 4 * m, hipMemcpyDeviceToHost, st()));
+        HIPC(ctx, hipStreamSynchronize(st()));
+      }
+      std::vector<HubChunk> chunk;
+      std::vector<int64_t> hbkv(m + 1, 0), bkhub;
       std::vector<HubBucket> bmeta;
       int64_t pos = 0, ncbo = 0, max_nch = 0;
       for (int64_t i = 0; i < m; ++i) {
         const int64_t e0 = ranges[2 * i], e1 = ranges[2 * i + 1], d = e1 - e0;
-        chunk0[i] = (int64_t)chunk.size() / 3;
-        for (int64_t e = e0; e < e1; e += CH) {
-          chunk.push_back(i);
-          chunk.push_back(e);
-          chunk.push_back(pos + (e - e0));
-        }
         int64_t P = (d + BK - 1) / BK;
         if (P < 1) P = 1;
         if (P > hub_max_buckets()) P = hub_max_buckets();
         hbkv[i + 1] = hbkv[i] + P;
-        const int64_t nch = (int64_t)chunk.size() / 3 - chunk0[i];
+        const int64_t nch = (d + CH - 1) / CH;
+        for (int64_t c = 0; c < nch; ++c) {
+          const int64_t ce = e0 + c * CH;
+          chunk.push_back(HubChunk{ce, std::min(ce + CH, e1), pos + c * CH, ncbo + c * (P + 1),
+                                   (int32_t)i, (int32_t)P, hubv[i], hgid[i]});
+        }
         for (int64_t j = 0; j < P; ++j) {
           bkhub.push_back(i);
           bmeta.push_back(HubBucket{pos, ncbo, (int32_t)i, hubv[i], (int32_t)j, (int32_t)P,
-                                    (int32_t)nch, 0});
+                                    (int32_t)nch, hgid[i]});
         }
         pos += d;
-        cbo0[i] = ncbo;
         ncbo += nch * (P + 1);
         max_nch = std::max(max_nch, nch);
       }
-      chunk0[m] = (int64_t)chunk.size() / 3;
       // segmented form unless a hub has more chunks than the aggregate's LDS
       // segment table holds (degree above hub_seg_max() chunks)
       hub.segmented = (tune.hub_segmented && max_nch <= hub_seg_max()) ? 1 : 0;
       hub.m = m;
-      hub.nchunks = (int64_t)chunk.size() / 3;
+      hub.nchunks = (int64_t)chunk.size();
       hub.nbuckets = hbkv[m];
       hub.ehub = pos;
       const size_t sb = (size_t)slot_bytes(g.mode);
       const size_t e = (size_t)std::max<int64_t>(pos, 1);
-      HIPC(ctx, hchunk.ensure(8 * std::max<size_t>(chunk.size(), 1)));
+      HIPC(ctx, hchunk.ensure(sizeof(HubChunk) * std::max<size_t>(chunk.size(), 1)));
+      HIPC(ctx, hgin.ensure(sizeof(HubGain) * m));
       HIPC(ctx, hv.ensure(4 * m));
       HIPC(ctx, hbk.ensure(8 * (m + 1)));
       HIPC(ctx, hbkhub.ensure(8 * hub.nbuckets));
@@ -519,8 +535,6 @@ struct Engine {
         HIPC(ctx, hbktw.ensure(sb * e));
       }
       HIPC(ctx, hbcount.ensure(8 * hub.nbuckets));
-      HIPC(ctx, hchunk0.ensure(8 * (m + 1)));
-      HIPC(ctx, hcbo0.ensure(8 * m));
       HIPC(ctx, hcbo.ensure(4 * std::max<int64_t>(ncbo, 1)));
       HIPC(ctx, hgalloc.ensure(8));
       HIPC(ctx, hboff.ensure(8 * (hub.nbuckets + 1)));
@@ -532,17 +546,17 @@ struct Engine {
       HIPC(ctx, hgk.ensure(4 * 2 * e));
       HIPC(ctx, hgv.ensure(sb * 2 * e));
       HIPC(ctx, hscan.ensure(DevScan::temp_bytes(hub.nbuckets + 1)));
-      HIPC(ctx, hipMemcpyAsync(hchunk.p, chunk.data(), 8 * chunk.size(), hipMemcpyHostToDevice, st()));
+      HIPC(ctx, hipMemcpyAsync(hchunk.p, chunk.data(), sizeof(HubChunk) * chunk.size(),
+                               hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hv.p, hperm, 4 * m, hipMemcpyDeviceToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hbk.p, hbkv.data(), 8 * (m + 1), hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipMemcpyAsync(hbkhub.p, bkhub.data(), 8 * hub.nbuckets, hipMemcpyHostToDevice, st()));
       HIPC(ctx, hbmeta.ensure(sizeof(HubBucket) * hub.nbuckets));
       HIPC(ctx, hipMemcpyAsync(hbmeta.p, bmeta.data(), sizeof(HubBucket) * hub.nbuckets,
                                hipMemcpyHostToDevice, st()));
-      HIPC(ctx, hipMemcpyAsync(hchunk0.p, chunk0.data(), 8 * (m + 1), hipMemcpyHostToDevice, st()));
-      HIPC(ctx, hipMemcpyAsync(hcbo0.p, cbo0.data(), 8 * m, hipMemcpyHostToDevice, st()));
       HIPC(ctx, hipStreamSynchronize(st()));
-      hub.chunk = hchunk.as<int64_t>();
+      hub.chunks = hchunk.as<HubChunk>();
+      hub.gin = hgin.as<HubGain>();
       hub.hub_v = hv.as<vid>();
       hub.hub_bk = hbk.as<int64_t>();
       hub.bk_hub = hbkhub.as<int64_t>();
@@ -563,13 +577,12 @@ struct Engine {
       hub.gk = hgk.as<vid>();
       hub.gv = hgv.p;
       hub.scan_tmp = hscan.p;
-      hub.hub_chunk0 = hchunk0.as<int64_t>();
-      hub.hub_cbo0 = hcbo0.as<int64_t>();
       hub.cbo = hcbo.as<int32_t>();
       hub.galloc = hgalloc.as<unsigned long long>();
       hub.bmeta = hbmeta.as<HubBucket>();
       hub.lds_slots = Hlds;
       hub.agg_bs = tune.hub_agg_threads;
+      hub.count_bs = tune.hub_count_threads;
     }
     return MGL_OK;
   }
@@ -2331,7 +2344,8 @@ mgl_status mgl_session_coarsen(mgl_session* s, int64_t* n2, int64_t* nnz2, int64
 // ---------------------------------------------------------------------------
 // Tuning file (text, written by mgl_autotune): lines starting with '#' and
 // empty lines are skipped; "hub_bucket N", "hub_bucket_unclamped N",
-// "hub_segmented 0|1", "hub_agg_threads 256|512|1024" and "hub_agg_per_cu 1..8"
+// "hub_segmented 0|1", "hub_agg_threads 256|512|1024", "hub_count_threads
+// 256|512|1024" and "hub_agg_per_cu 1..8"
 // set the hub path; "table_check 0" accepts tables smaller than their bin's rows
 // (a move that overflows one then fails with MGL_DEVICE_ERROR);
 // "hub_stream 0" keeps the hub path on the main stream; "hub_fork_upper N"
@@ -2381,6 +2395,9 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
     } else if (sscanf(p, "hub_agg_threads %d %c", &hs, &extra) == 1) {
       if (hs != 256 && hs != 512 && hs != 1024) bad = "hub_agg_threads";
       tu.hub_agg_threads = hs;
+    } else if (sscanf(p, "hub_count_threads %d %c", &hs, &extra) == 1) {
+      if (hs != 256 && hs != 512 && hs != 1024) bad = "hub_count_threads";
+      tu.hub_count_threads = hs;
     } else if (sscanf(p, "hub_agg_per_cu %d %c", &hs, &extra) == 1) {
       if (hs < 1 || hs > 8) bad = "hub_agg_per_cu";
       tu.hub_agg_per_cu = hs;
@@ -2411,12 +2428,17 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
         FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d threads %d", path, i, b.s);
       if (b.size < 1 || (tu.table_check && b.upper > b.size))
         FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table too small", path, i);
+    } else if (b.shape == SHAPE_PHASE) {
+      if (b.s != 512 && b.s != 1024)
+        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d threads %d", path, i, b.s);
+      if (b.size < 1) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table size", path, i);
     } else if (b.shape == SHAPE_LANE) {
       if (b.upper > 16) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: lane bin above 16", path);
     } else if (b.shape != SHAPE_GLOBAL) {
       FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d shape", path, i);
     }
-    if (b.shape == SHAPE_GROUP || b.shape == SHAPE_BLOCK || b.shape == SHAPE_LANE) {
+    if (b.shape == SHAPE_GROUP || b.shape == SHAPE_BLOCK || b.shape == SHAPE_LANE ||
+        b.shape == SHAPE_PHASE) {
       // every mode instantiates the same shapes
       char kn[128];
       if (b.upper == INT64_MAX || move_kernel_name(MODE_UNIT, 1, b, kn, (int)sizeof(kn)) != 0)

@@ -27,14 +27,14 @@
 #include "mgl_kernels.h"
 #include "mgl_tables.h"
 
-#define HUB_CHUNK 8192   // 32 entries per thread of the count step
+#define HUB_CHUNK 8192   // entries of one count workgroup (8 to 32 per thread)
 #define HUB_T 256
 #define HUB_ITEMS (HUB_CHUNK / HUB_T)
 #define HUB_MAXP 1024
 #define HUB_SEG_MAX 512   // chunks per hub in the segmented form
 // Aggregate workgroups: BS threads (HubPlan::agg_bs: 256, 512 or 1024),
 // 6,144 / BS candidates per thread on the owner path.
-#define HUB_AGG_CAP 6144
+#define HUB_AGG_CAP 4096
 
 __device__ __forceinline__ uint32_t hub_bucket(vid key, uint32_t P) {
   return (uint32_t)(((uint64_t)((uint32_t)key * 2654435761u) * P) >> 32);
@@ -71,15 +71,16 @@ __device__ __forceinline__ void tab_insert(vid* K, ST* V, AT* Av, vid key, ST w,
   }
 }
 
-template <class T>
-__device__ __forceinline__ T bsum256(T v, T* red) {
+// Sum over a workgroup of NT threads (red holds NT / 64 entries).
+template <int NT, class T>
+__device__ __forceinline__ T bsum_t(T v, T* red) {
   v = group_sum<64>(v);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   T s = 0;
 #pragma unroll
-  for (int i = 0; i < HUB_T / 64; ++i) s += red[i];
+  for (int i = 0; i < NT / 64; ++i) s += red[i];
   return s;
 }
 
@@ -90,21 +91,16 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
   typedef typename M::K KT;
   __shared__ unsigned int hist[HUB_MAXP];
   __shared__ KT red[HUB_T / 64];
-  const int64_t ch = blockIdx.x;
-  const int64_t i = H.chunk[3 * ch];
-  const int64_t e0 = H.chunk[3 * ch + 1];
-  const int64_t pos0 = H.chunk[3 * ch + 2];
-  const vid v = H.hub_v[i];
-  const int g = A.gid ? A.gid[v] : 0;
-  if (!A.gs[g].active) return;
-  const int64_t rend = A.off[v + 1];
-  const int64_t e1 = (e0 + HUB_CHUNK < rend) ? e0 + HUB_CHUNK : rend;
-  const uint32_t P = (uint32_t)(H.hub_bk[i + 1] - H.hub_bk[i]);
+  const HubChunk hc = H.chunks[blockIdx.x];
+  const int64_t i = hc.hub, e0 = hc.e0, e1 = hc.e1, pos0 = hc.pos0;
+  const vid v = hc.v;
+  if (!A.gs[hc.g].active) return;
+  const uint32_t P = (uint32_t)hc.P;
   for (uint32_t b = threadIdx.x; b < P; b += HUB_T) hist[b] = 0;
   __syncthreads();
   const Rec<AT>* R = (const Rec<AT>*)A.rec;
   const vid vp = A.rpos ? A.rpos[v] : v;
-  const vid ci = R[vp].c;
+  const vid ci = H.gin[i].ci;
   const bool v1 = A.version == 1;
   KT eici = 0, selfw = 0;
   for (int64_t e = e0 + threadIdx.x; e < e1; e += HUB_T) {
@@ -127,8 +123,8 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
       }
     }
   }
-  eici = bsum256(eici, red);
-  selfw = bsum256(selfw, red);
+  eici = bsum_t<HUB_T>(eici, red);
+  selfw = bsum_t<HUB_T>(selfw, red);
   if (threadIdx.x == 0) {
     if (eici != (KT)0) hatom_add(&((KT*)H.hub_eici)[i], eici);
     if (selfw != (KT)0) hatom_add(&((KT*)H.hub_self)[i], selfw);
@@ -139,56 +135,73 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count(MoveArgs A, HubPlan H) {
 }
 
 // ---- segmented form: head and count --------------------------------------------
-// Per pass: zero the hubs' own-community sums and the HBM-table pool.
-__global__ void k_hub_head(HubPlan H) {
+// Per pass: zero the hubs' own-community sums and the HBM-table pool, and
+// record each hub's gain inputs (community, total, community total without it).
+template <class M>
+__global__ void k_hub_head(MoveArgs A, HubPlan H) {
+  typedef typename M::K KT;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *H.galloc = 0;
   if (i < H.m) {
     ((int64_t*)H.hub_eici)[i] = 0;  // 8-byte sums (int64 or double zero)
     ((int64_t*)H.hub_self)[i] = 0;
+    const vid v = H.hub_v[i];
+    const vid ci = A.c[v];
+    const KT ki = ((const KT*)A.k)[v];
+    const KT acik = ((const KT*)A.a)[ci] - ki;
+    HubGain hg;
+    __builtin_memcpy(&hg.ki, &ki, 8);
+    __builtin_memcpy(&hg.acik, &acik, 8);
+    hg.ci = ci;
+    hg.pad = 0;
+    H.gin[i] = hg;
   }
+}
+template <class KT>
+__device__ __forceinline__ KT gin_bits(int64_t x) {
+  KT r;
+  __builtin_memcpy(&r, &x, 8);
+  return r;
 }
 
 // Gathers c[u] once per entry of a chunk, sums the own community (eici,
 // self-loop), and writes the chunk's candidates into its region of tmp grouped
 // by bucket (counting sort in LDS); cbo receives the chunk's bucket offsets.
-template <class M, class AT>
-__global__ __launch_bounds__(HUB_T) void k_hub_count_seg(MoveArgs A, HubPlan H) {
+template <class M, class AT, int CT>
+__global__ __launch_bounds__(CT) void k_hub_count_seg(MoveArgs A, HubPlan H) {
   typedef typename M::S ST;
   typedef typename M::K KT;
+  constexpr int CI = HUB_CHUNK / CT;  // entries per thread
   __shared__ unsigned int hist[HUB_MAXP + 1];
-  __shared__ unsigned int wsum[HUB_T / 64];
-  __shared__ KT red[HUB_T / 64];
-  const int64_t ch = blockIdx.x;
-  const int64_t i = H.chunk[3 * ch];
-  const int64_t e0 = H.chunk[3 * ch + 1];
-  const int64_t pos0 = H.chunk[3 * ch + 2];
-  const vid v = H.hub_v[i];
-  const int g = A.gid ? A.gid[v] : 0;
-  if (!A.gs[g].active) return;
-  const int64_t rend = A.off[v + 1];
-  const int64_t e1 = (e0 + HUB_CHUNK < rend) ? e0 + HUB_CHUNK : rend;
-  const uint32_t P = (uint32_t)(H.hub_bk[i + 1] - H.hub_bk[i]);
-  for (uint32_t b = threadIdx.x; b <= P; b += HUB_T) hist[b] = 0;
-  const Rec<AT>* R = (const Rec<AT>*)A.rec;
-  const vid vp = A.rpos ? A.rpos[v] : v;
-  const vid ci = R[vp].c;
-  const bool v1 = A.version == 1;
-  vid key[HUB_ITEMS];
-  Rec<AT> r[HUB_ITEMS];
+  __shared__ unsigned int wsum[CT / 64];
+  __shared__ KT red[CT / 64];
+  // the chunk record is the only load the rest waits on: the columns, the
+  // graph's state and the hub's community are requested together after it
+  const HubChunk hc = H.chunks[blockIdx.x];
+  const int64_t i = hc.hub, e0 = hc.e0, e1 = hc.e1, pos0 = hc.pos0;
+  const vid v = hc.v;
+  const uint32_t P = (uint32_t)hc.P;
+  vid key[CI];
+  Rec<AT> r[CI];
 #pragma unroll
-  for (int j = 0; j < HUB_ITEMS; ++j) {
-    const int64_t e = e0 + (int64_t)j * HUB_T + threadIdx.x;
+  for (int j = 0; j < CI; ++j) {
+    const int64_t e = e0 + (int64_t)j * CT + threadIdx.x;
     key[j] = e < e1 ? colload(A.col, e) : MGL_EMPTY;  // neighbour position first
   }
+  const vid ci = H.gin[i].ci;
+  const vid vp = A.rpos ? A.rpos[v] : v;
+  if (!A.gs[hc.g].active) return;
+  for (uint32_t b = threadIdx.x; b <= P; b += CT) hist[b] = 0;
+  const Rec<AT>* R = (const Rec<AT>*)A.rec;
+  const bool v1 = A.version == 1;
 #pragma unroll
-  for (int j = 0; j < HUB_ITEMS; ++j)
+  for (int j = 0; j < CI; ++j)
     if (key[j] != MGL_EMPTY) r[j] = R[key[j]];
   __syncthreads();  // hist cleared
   KT eici = 0, selfw = 0;
-  uint32_t slot[HUB_ITEMS];  // bucket << 16 | rank within the chunk's bucket
+  uint32_t slot[CI];  // bucket << 16 | rank within the chunk's bucket
 #pragma unroll
-  for (int j = 0; j < HUB_ITEMS; ++j) {
+  for (int j = 0; j < CI; ++j) {
     if (key[j] == MGL_EMPTY) continue;
     const vid u = key[j];
     const vid cj = r[j].c;
@@ -200,7 +213,7 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count_seg(MoveArgs A, HubPlan H) 
     } else {
       key[j] = MGL_EMPTY;
       if (cj == ci) {
-        const ST wv = wload<M>(A.w, e0 + (int64_t)j * HUB_T + threadIdx.x);
+        const ST wv = wload<M>(A.w, e0 + (int64_t)j * CT + threadIdx.x);
         if (u != vp) eici += (KT)wv;
         else selfw += (KT)wv;
       }
@@ -232,21 +245,21 @@ __global__ __launch_bounds__(HUB_T) void k_hub_count_seg(MoveArgs A, HubPlan H) 
     if (b < P) hist[b] = off;
     off += c4[k];
   }
-  if (threadIdx.x == HUB_T - 1) hist[P] = off;  // candidates of the chunk
+  if (threadIdx.x == CT - 1) hist[P] = off;  // candidates of the chunk
   __syncthreads();
-  const int64_t cq = H.hub_cbo0[i] + (ch - H.hub_chunk0[i]) * (int64_t)(P + 1);
-  for (uint32_t b = threadIdx.x; b <= P; b += HUB_T) H.cbo[cq + b] = (int32_t)hist[b];
+  const int64_t cq = hc.cq;
+  for (uint32_t b = threadIdx.x; b <= P; b += CT) H.cbo[cq + b] = (int32_t)hist[b];
 #pragma unroll
-  for (int j = 0; j < HUB_ITEMS; ++j) {
+  for (int j = 0; j < CI; ++j) {
     if (key[j] == MGL_EMPTY) continue;
     const int64_t dst = pos0 + hist[slot[j] >> 16] + (slot[j] & 0xFFFFu);
     H.tmp_c[dst] = key[j];
     ((AT*)H.tmp_a)[dst] = r[j].a;
     if (M::HAS_W)
-      ((ST*)H.tmp_w)[dst] = wload<M>(A.w, e0 + (int64_t)j * HUB_T + threadIdx.x);
+      ((ST*)H.tmp_w)[dst] = wload<M>(A.w, e0 + (int64_t)j * CT + threadIdx.x);
   }
-  eici = bsum256(eici, red);
-  selfw = bsum256(selfw, red);
+  eici = bsum_t<CT>(eici, red);
+  selfw = bsum_t<CT>(selfw, red);
   if (threadIdx.x == 0) {
     if (eici != (KT)0) hatom_add(&((KT*)H.hub_eici)[i], eici);
     if (selfw != (KT)0) hatom_add(&((KT*)H.hub_self)[i], selfw);
@@ -259,16 +272,11 @@ __global__ __launch_bounds__(HUB_T) void k_hub_scatter(MoveArgs A, HubPlan H) {
   typedef typename M::S ST;
   __shared__ unsigned int cnt[HUB_MAXP];
   __shared__ int64_t base[HUB_MAXP];
-  const int64_t ch = blockIdx.x;
-  const int64_t i = H.chunk[3 * ch];
-  const int64_t e0 = H.chunk[3 * ch + 1];
-  const int64_t pos0 = H.chunk[3 * ch + 2];
-  const vid v = H.hub_v[i];
-  const int g = A.gid ? A.gid[v] : 0;
-  if (!A.gs[g].active) return;
-  const int64_t rend = A.off[v + 1];
-  const int64_t len = ((e0 + HUB_CHUNK < rend) ? e0 + HUB_CHUNK : rend) - e0;
-  const uint32_t P = (uint32_t)(H.hub_bk[i + 1] - H.hub_bk[i]);
+  const HubChunk hc = H.chunks[blockIdx.x];
+  const int64_t i = hc.hub, pos0 = hc.pos0;
+  if (!A.gs[hc.g].active) return;
+  const int64_t len = hc.e1 - hc.e0;
+  const uint32_t P = (uint32_t)hc.P;
   for (uint32_t b = threadIdx.x; b < P; b += HUB_T) cnt[b] = 0;
   __syncthreads();
   vid key[HUB_ITEMS];
@@ -391,10 +399,18 @@ struct BucketSegs {
 // first try an LDS table that is scanned at the end (4-byte sums); if their
 // distinct communities overfill it, or the sums take 8 bytes, an HBM table of
 // 2*cnt slots from the pass's pool (H.galloc) is used.
+// The gain inputs of a bucket's hub, loaded by the caller at its start.
+struct HubIn {
+  double mass;     // 2m of the hub's graph
+  int64_t mass_i;  // 2m in integer units (exact modes)
+  int64_t ki, acik, eici;  // bits of the mode's total type
+  vid ci;
+};
+
 template <class M, class AT, int BS, int IT, bool CNT, class SRC>
 __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubPlan& H,
-                                                   uint32_t Hmax, int64_t b, int64_t i, vid v,
-                                                   int g, const SRC& src, int64_t cnt,
+                                                   uint32_t Hmax, int64_t b, const HubIn& in,
+                                                   const SRC& src, int64_t cnt,
                                                    unsigned char* smem) {
   typedef typename M::S ST;
   typedef typename M::K KT;
@@ -406,18 +422,15 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
   __shared__ int ovf;
   __shared__ unsigned int ncre;
   __shared__ unsigned long long gbase;
-  const vid ci = A.c[v];
+  const vid ci = in.ci;
   const bool small = A.tie_small ? true : (A.up != 0);
   double best = 0.0;
   vid dst = ci;
   int occ = 0;
   ProbeCount pc;
   if (cnt > 0) {
-    const KT ki = ((const KT*)A.k)[v];
-    const KT* a = (const KT*)A.a;
-    const KT aci_k = a[ci] - ki;
-    const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k,
-                                    ((const KT*)H.hub_eici)[i]);
+    const GainIn G = gain_in<M, AT>(in.mass, in.mass_i, gin_bits<KT>(in.ki),
+                                    gin_bits<KT>(in.acik), gin_bits<KT>(in.eici));
     const AT* ba = (const AT*)src.a;
     if (cnt <= (int64_t)IT * BS && (uint64_t)cnt * 8 <= (uint64_t)Hmax * 7) {
       const uint32_t Ht = hub_table_size(cnt, Hmax);
@@ -544,8 +557,21 @@ __device__ __forceinline__ void hub_aggregate_body(const MoveArgs& A, const HubP
 }
 
 // One workgroup per bucket, after the scatter (contiguous buckets).
+// Gain inputs of hub i of graph g.
+__device__ __forceinline__ HubIn hub_in(const MoveArgs& A, const HubPlan& H, int64_t i, int g) {
+  const HubGain hg = H.gin[i];
+  HubIn in;
+  in.mass = A.gs[g].mass;
+  in.mass_i = A.gs[g].mass_i;
+  in.ki = hg.ki;
+  in.acik = hg.acik;
+  in.eici = ((const int64_t*)H.hub_eici)[i];  // 8-byte sums: the bits of KT
+  in.ci = hg.ci;
+  return in;
+}
+
 template <class M, class AT, int BS, int IT, bool CNT>
-__global__ __launch_bounds__(BS) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hmax) {
+__global__ __launch_bounds__(BS, BS >= 512 ? 8 : 1) void k_hub_aggregate(MoveArgs A, HubPlan H, uint32_t Hmax) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int64_t b = blockIdx.x;
   const int64_t i = H.bk_hub[b];
@@ -554,7 +580,8 @@ __global__ __launch_bounds__(BS) void k_hub_aggregate(MoveArgs A, HubPlan H, uin
   if (!A.gs[g].active) return;
   const int64_t s0 = H.boff[b], s1 = H.boff[b + 1];
   BucketContig src{H.bkt_c, H.bkt_a, H.bkt_w, s0};
-  hub_aggregate_body<M, AT, BS, IT, CNT>(A, H, Hmax, b, i, v, g, src, s1 - s0, smem);
+  const HubIn in = hub_in(A, H, i, g);
+  hub_aggregate_body<M, AT, BS, IT, CNT>(A, H, Hmax, b, in, src, s1 - s0, smem);
 }
 
 // Block-wide exclusive scan of one value per thread (BS threads).
@@ -578,7 +605,7 @@ __device__ __forceinline__ int32_t exscan_block(int32_t x, int32_t* wsum /* [BS 
 // One workgroup per bucket in the segmented form: the bucket's segment in each
 // chunk of its hub is located from the chunks' bucket offsets.
 template <class M, class AT, int BS, int IT, bool CNT>
-__global__ __launch_bounds__(BS) void k_hub_aggregate_seg(MoveArgs A, HubPlan H, uint32_t Hmax) {
+__global__ __launch_bounds__(BS, BS >= 512 ? 8 : 1) void k_hub_aggregate_seg(MoveArgs A, HubPlan H, uint32_t Hmax) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int32_t pre[HUB_SEG_MAX + 1];
   __shared__ int32_t srcs[HUB_SEG_MAX];
@@ -586,9 +613,10 @@ __global__ __launch_bounds__(BS) void k_hub_aggregate_seg(MoveArgs A, HubPlan H,
   const int64_t b = blockIdx.x;
   const HubBucket hb = H.bmeta[b];  // one (scalar) load: the bucket's fixed layout
   const int64_t i = hb.hub;
-  const vid v = hb.v;
-  const int g = A.gid ? A.gid[v] : 0;
-  if (!A.gs[g].active) return;
+  // everything below depends on the bucket record only: the segment offsets,
+  // the hub's gain inputs and the graph's state are requested together
+  const HubIn in = hub_in(A, H, i, hb.g);
+  const int act = A.gs[hb.g].active;
   const int64_t lb = hb.lb;
   const int64_t P = hb.P;
   const int nch = hb.nch;
@@ -611,6 +639,7 @@ __global__ __launch_bounds__(BS) void k_hub_aggregate_seg(MoveArgs A, HubPlan H,
     }
     tot += len[k];
   }
+  if (!act) return;  // uniform: the bucket's graph has finished
   int32_t ex = exscan_block<BS>(tot, wsum);
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
@@ -622,7 +651,7 @@ __global__ __launch_bounds__(BS) void k_hub_aggregate_seg(MoveArgs A, HubPlan H,
   __syncthreads();
   const int64_t cnt = pre[nch];
   BucketSegs src{H.tmp_c, H.tmp_a, H.tmp_w, base, pre, srcs, nch};
-  hub_aggregate_body<M, AT, BS, IT, CNT>(A, H, Hmax, b, i, v, g, src, cnt, smem);
+  hub_aggregate_body<M, AT, BS, IT, CNT>(A, H, Hmax, b, in, src, cnt, smem);
 }
 
 // ---- finalize -----------------------------------------------------------------
@@ -663,7 +692,7 @@ static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t
   const bool cnt = A.stats || A.probes;
   const size_t lds = (sizeof(ST) == 4) ? 8 * (size_t)Hlds
                                        : ((4 * (size_t)Hlds + 15) / 16) * 16 + sizeof(ST) * (size_t)Hlds;
-  k_hub_head<<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(H);
+  k_hub_head<M><<<(unsigned)((H.m + 255) / 256), 256, 0, st>>>(A, H);
   const unsigned nbk = (unsigned)H.nbuckets;
 #define AGG_(K_, BS_)                                                                    \
   if (H.agg_bs == BS_) {                                                                 \
@@ -671,7 +700,10 @@ static int hub_t(const MoveArgs& A, const HubPlan& H, uint32_t Hlds, hipStream_t
     else K_<M, AT, BS_, HUB_AGG_CAP / BS_, false><<<nbk, BS_, lds, st>>>(A, H, Hlds);     \
   }
   if (H.segmented) {
-    k_hub_count_seg<M, AT><<<(unsigned)H.nchunks, HUB_T, 0, st>>>(A, H);
+    if (H.count_bs == 512) k_hub_count_seg<M, AT, 512><<<(unsigned)H.nchunks, 512, 0, st>>>(A, H);
+    else if (H.count_bs == 1024)
+      k_hub_count_seg<M, AT, 1024><<<(unsigned)H.nchunks, 1024, 0, st>>>(A, H);
+    else k_hub_count_seg<M, AT, 256><<<(unsigned)H.nchunks, 256, 0, st>>>(A, H);
     AGG_(k_hub_aggregate_seg, 256) AGG_(k_hub_aggregate_seg, 512) AGG_(k_hub_aggregate_seg, 1024)
   } else {
     launch_fill_bytes0(H.bcount, 8 * (size_t)H.nbuckets, st);

@@ -11,12 +11,16 @@
 //   SHAPE_GLOBAL: hub path (mgl_hub.hip): rows split into chunks, candidates
 //                 bucketed by hash, one workgroup per bucket
 //   SHAPE_LANE  : one lane per vertex, the row merged in registers (degree <= 16)
-enum Shape { SHAPE_NONE = 0, SHAPE_GROUP = 1, SHAPE_BLOCK = 2, SHAPE_GLOBAL = 3, SHAPE_LANE = 4 };
+//   SHAPE_PHASE : one workgroup of BS threads per vertex, the row's records kept
+//                 in registers and aggregated in phases (hash ranges) in an LDS
+//                 hash of `size` slots, which may be smaller than the row
+enum Shape { SHAPE_NONE = 0, SHAPE_GROUP = 1, SHAPE_BLOCK = 2, SHAPE_GLOBAL = 3, SHAPE_LANE = 4,
+             SHAPE_PHASE = 5 };
 struct BinSpec {
   int64_t upper;  // inclusive degree bound (INT64_MAX for the last bin)
   int shape;
-  int s;          // lanes per vertex (GROUP) or threads per block (BLOCK)
-  int size;       // GROUP: log2 of slots per vertex; BLOCK: slots per vertex
+  int s;          // lanes per vertex (GROUP) or threads per block (BLOCK, PHASE)
+  int size;       // GROUP: log2 of slots per vertex; BLOCK, PHASE: slots per vertex
 };
 #define MGL_MAX_BINS 16
 
@@ -99,11 +103,33 @@ struct HubBucket {
   int32_t lb;    // bucket index within the hub
   int32_t P;     // buckets of the hub
   int32_t nch;   // chunks of the hub
+  int32_t g;     // graph of the hub (batches)
+};
+// One chunk of a hub's row, fixed for a level: CSR entries [e0, e1), their
+// compact position in tmp, the chunk's bucket offsets in cbo (segmented form)
+// and its hub.
+struct HubChunk {
+  int64_t e0, e1;
+  int64_t pos0;  // tmp position of entry e0
+  int64_t cq;    // first of the chunk's P + 1 entries in cbo
+  int32_t hub;   // hub index
+  int32_t P;     // buckets of the hub
+  vid v;         // hub vertex
+  int32_t g;     // graph of the hub
+};
+// Per hub and pass (k_hub_head): the inputs of its gains, so that the hub
+// kernels load them in one step instead of a chain of dependent loads. ki and
+// acik hold the bits of the mode's total type (int64 or double).
+struct HubGain {
+  int64_t ki;    // the hub's total
+  int64_t acik;  // its community's total without the hub
+  vid ci;        // its community
   int32_t pad;
 };
 struct HubPlan {
   int64_t m = 0, nchunks = 0, nbuckets = 0, ehub = 0;
-  const int64_t* chunk = nullptr;   // [3*nchunks]: hub index, first CSR entry, compact position
+  const HubChunk* chunks = nullptr; // [nchunks]
+  HubGain* gin = nullptr;           // [m]
   const vid* hub_v = nullptr;       // [m]
   const int64_t* hub_bk = nullptr;  // [m+1] bucket ranges
   const int64_t* bk_hub = nullptr;  // [nbuckets] owning hub
@@ -130,12 +156,11 @@ struct HubPlan {
   // chunk's P_i + 1 bucket offsets; the aggregate reads a bucket's segments
   // from every chunk of its hub, so no scatter pass and no global scan.
   int segmented = 0;
-  const int64_t* hub_chunk0 = nullptr;  // [m+1] first chunk of each hub
-  const int64_t* hub_cbo0 = nullptr;    // [m] first cbo entry of each hub
   int32_t* cbo = nullptr;               // [sum over hubs of chunks x (P_i + 1)]
   unsigned long long* galloc = nullptr; // [1] HBM-table slots handed out in this pass
   const HubBucket* bmeta = nullptr;     // [nbuckets] (segmented form)
   int agg_bs = 1024;                    // threads per aggregate workgroup: 256, 512 or 1024
+  int count_bs = 256;                   // threads per count workgroup (segmented form)
 };
 int launch_hub(int mode, int rec32, const MoveArgs& A, const HubPlan& H, hipStream_t st);
 void hub_allow_lds(int bytes);
@@ -144,6 +169,8 @@ int hub_max_buckets();
 int hub_seg_max();  // chunks per hub of the segmented form
 int hub_agg_cap();  // candidates of one bucket on the aggregate's owner path
 void launch_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out, hipStream_t st);
+// out[i] = src[idx[i]] for i < n
+void launch_gather_i32(const int32_t* src, const vid* idx, int64_t n, int32_t* out, hipStream_t st);
 
 // ---- per-vertex row reductions (vertex weights, modularity sum_in) ----------
 struct RowArgs {

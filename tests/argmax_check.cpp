@@ -120,6 +120,12 @@ int main(int argc, char** argv) {
         x.e = (int32_t)logu(1, emax);
         x.a = (int32_t)uni(0, D - ki);
       }
+      // every fourth case: most candidates of weight 1 (a unit-weight row of
+      // distinct neighbour communities), totals spread log-uniformly
+      if ((t & 3) == 3 && uni(0, 4) != 0) {
+        x.e = 1;
+        x.a = (int32_t)logu(1, D - ki > 1 ? D - ki : 1);
+      }
       cs.push_back(x);
     }
     const double mass = (double)D;
