@@ -228,12 +228,14 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-template <int BS>
+// LEAD = false: rg and rd have not been used before in the kernel, so the
+// barrier that orders their reuse is left out.
+template <int BS, bool LEAD = true>
 __device__ __forceinline__ void block_argmax(double& bg, vid& bd, bool small, double* rg,
                                              vid* rd) {
   group_argmax<64>(bg, bd, small);
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  __syncthreads();
+  if constexpr (LEAD) __syncthreads();
   if (l == 0) { rg[w] = bg; rd[w] = bd; }
   __syncthreads();
 #pragma unroll

@@ -239,7 +239,7 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   }
   double best = am.best;
   vid dst = am.dst;
-  block_argmax<BS>(best, dst, small, rg, rd);
+  block_argmax<BS, false>(best, dst, small, rg, rd);  // rg, rd unused before
   if (CNT && A.stats) cnt = block_sum<BS>(cnt, rc);
   if (threadIdx.x == 0) {
     A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   vid dst = ci;
   int cnt = 0;
   own_gains<M, AT, IT, TB>(tb, own, oa, G, small, best, dst, cnt);
-  block_argmax<BS>(best, dst, small, rg, rd);
+  block_argmax<BS, false>(best, dst, small, rg, rd);  // rg, rd unused before
   if (CNT && A.stats) cnt = block_sum<BS>(cnt, rc);
   if (threadIdx.x == 0) {
     A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
@@ -610,7 +610,7 @@ __global__ __launch_bounds__(BS, 4) void k_move_block_ph(MoveArgs A, const vid* 
   if constexpr (CNT) flush_probes(A.probes, pc);
   double best = am.best;
   vid dst = am.dst;
-  block_argmax<BS>(best, dst, small, rg, rd);
+  block_argmax<BS, false>(best, dst, small, rg, rd);  // rg, rd unused before
   if (CNT && A.stats) cnt = block_sum<BS>(cnt, rc);
   if (threadIdx.x == 0) {
     A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
