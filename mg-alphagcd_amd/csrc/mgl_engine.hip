@@ -124,6 +124,30 @@ struct DBuf {
   template <class T> T* as() const { return (T*)p; }
 };
 
+// Pinned host buffer owned by the engine: the small reads of device state
+// between launches (graph states, the error word, the active count) land here
+// in one DMA each, without a pageable bounce buffer.
+struct HBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  HBuf() = default;
+  HBuf(const HBuf&) = delete;
+  HBuf& operator=(const HBuf&) = delete;
+  ~HBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t ensure(size_t b) {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = b;
+    else p = nullptr;
+    return e;
+  }
+};
+
 static size_t wsize(int mode) {  // bytes of one stored edge weight
   switch (mode) {
     case MODE_UNIT: return 0;
@@ -272,6 +296,7 @@ struct Engine {
   // per-graph
   DBuf bgs, brun, btraj, bttraj, bkeep, bdelta, bgather, bcounts;
   std::vector<GState> hgs;
+  HBuf hpin;  // staging of the reads below
   // bins of the current level
   std::vector<int64_t> bin_n, bin_off;
   // hub plan of the current level (last bin)
@@ -313,14 +338,18 @@ struct Engine {
   int* nactive() { return (int*)(bcounts.as<unsigned long long>() + 3 * MGL_MAX_BINS); }
   // device word: set by a move kernel whose LDS table overflowed
   int* err_word() { return (int*)(bcounts.as<unsigned long long>() + 4 * MGL_MAX_BINS + 3); }
+  mgl_status err_overflow() {
+    FAIL(ctx, MGL_DEVICE_ERROR,
+         "an LDS hash table of a move kernel overflowed (the bin layout's table is smaller "
+         "than a row's distinct communities)");
+  }
   mgl_status check_err() {
-    int e = 0;
-    HIPC(ctx, hipMemcpyAsync(&e, err_word(), 4, hipMemcpyDeviceToHost, st()));
+    HIPC(ctx, hpin.ensure(16));
+    HIPC(ctx, hipMemcpyAsync(hpin.p, err_word(), 4, hipMemcpyDeviceToHost, st()));
     HIPC(ctx, hipStreamSynchronize(st()));
-    if (e)
-      FAIL(ctx, MGL_DEVICE_ERROR,
-           "an LDS hash table of a move kernel overflowed (the bin layout's table is smaller "
-           "than a row's distinct communities)");
+    int e;
+    memcpy(&e, hpin.p, 4);
+    if (e) return err_overflow();
     return MGL_OK;
   }
   LevelGraph& LG() { return lg[cur]; }
@@ -791,10 +820,22 @@ struct Engine {
     return MGL_OK;
   }
 
-  mgl_status read_gs() {
-    hgs.resize(G);
-    HIPC(ctx, hipMemcpyAsync(hgs.data(), bgs.p, sizeof(GState) * G, hipMemcpyDeviceToHost, st()));
+  // The graph states into hgs; with check_error also the error word (one
+  // synchronisation for both), reported as check_err does.
+  mgl_status read_gs(bool check_error = false) {
+    const size_t gb = sizeof(GState) * G;
+    HIPC(ctx, hpin.ensure(gb + 16));
+    HIPC(ctx, hipMemcpyAsync(hpin.p, bgs.p, gb, hipMemcpyDeviceToHost, st()));
+    if (check_error)
+      HIPC(ctx, hipMemcpyAsync((char*)hpin.p + gb, err_word(), 4, hipMemcpyDeviceToHost, st()));
     HIPC(ctx, hipStreamSynchronize(st()));
+    hgs.resize(G);
+    memcpy(hgs.data(), hpin.p, gb);
+    if (check_error) {
+      int e;
+      memcpy(&e, (char*)hpin.p + gb, 4);
+      if (e) return err_overflow();
+    }
     return MGL_OK;
   }
 
@@ -807,8 +848,10 @@ struct Engine {
   }
 
   mgl_status read_nactive(int& na) {
-    HIPC(ctx, hipMemcpyAsync(&na, nactive(), 4, hipMemcpyDeviceToHost, st()));
+    HIPC(ctx, hpin.ensure(16));
+    HIPC(ctx, hipMemcpyAsync(hpin.p, nactive(), 4, hipMemcpyDeviceToHost, st()));
     HIPC(ctx, hipStreamSynchronize(st()));
+    memcpy(&na, hpin.p, 4);
     return MGL_OK;
   }
 
@@ -1090,9 +1133,7 @@ static mgl_status copy_level_graph_to_host(mgl_ctx* ctx, LevelGraph& g, int64_t 
 // to coarsen, build the next level's union graph.
 mgl_status Engine::level_end(int level) {
   LevelGraph& g = LG();
-  mgl_status s = check_err();
-  if (s) return s;
-  s = read_gs();
+  mgl_status s = read_gs(true);
   if (s) return s;
   // trajectories: only the columns any graph used
   int tmax = 1;
@@ -2057,9 +2098,7 @@ mgl_status mgl_session_pass(mgl_session* s, double* q, int* accepted) {
   r = E.iteration(s->it);
   E.timing = false;
   if (r) return r;
-  r = E.check_err();
-  if (r) return r;
-  r = E.read_gs();
+  r = E.read_gs(true);
   if (r) return r;
   const GState& gs = E.hgs[0];
   const bool acc = E.o.fuse_modularity ? (gs.fix == 0 || gs.fix == 2) && gs.fix != 1
