@@ -386,7 +386,10 @@ struct Engine {
     launch_fill_bytes0(bcounts.p, 8 * (MGL_MAX_BINS * 4 + 4), st());
     c = bc.as<vid>(); p = bp.as<vid>(); nx = bn.as<vid>();
     if (!ev_ok) {
-      for (auto& x : ev) HIPC(ctx, hipEventCreate(&x));
+      // timing only (read after the pass's stream synchronisation): no
+      // system-scope fence at each record, which would write back and
+      // invalidate the caches between the move kernels of a pass
+      for (auto& x : ev) HIPC(ctx, hipEventCreateWithFlags(&x, hipEventDisableSystemFence));
       ev_ok = true;
     }
     return MGL_OK;
