@@ -203,12 +203,6 @@ static BinLayout layout_for(const mgl_ctx* ctx, const Tuning& tu, int mode, bool
   BinLayout L = tu.have_bins ? tu.bins : default_bins(ctx->lds_max, sb);
   for (int i = 0; i < L.nb; ++i) {
     BinSpec& b = L.b[i];
-    if (b.shape == SHAPE_PHASE) {
-      // any table size works (a full phase is split); it must fit the LDS
-      const int64_t cap = (int64_t)(ctx->lds_max - 2048) / sb;
-      if (b.size > cap) b.size = (int)cap;
-      b.size = prime_at_most(b.size);
-    }
     if (b.shape == SHAPE_BLOCK) {
       const bool owner = (b.upper + b.s - 1) / b.s <= 16;
       const int64_t cap =
@@ -288,8 +282,11 @@ struct Engine {
   DBuf bc, bp, bn, bk, ba, bvin, bperm, blab, bfin, bgid0, brec, bperm2, blist, bcnt2;
   int64_t nlist = -1;  // >= 0: per-vertex passes walk blist (active vertices of a batch)
   // record layout of the current level: rpos[v] = position of v's record,
-  // rinv = its inverse, colr = the level's columns as record positions
-  DBuf brpos, brinv, bcolr;
+  // rinv = its inverse; the move kernels read the level's rows placed in that
+  // order (offr: offsets by position, colr: columns as record positions, wr:
+  // their weights), so each bin's rows are contiguous; pdeg and pscan are
+  // the scratch of offr's scan
+  DBuf brpos, brinv, bcolr, boffr, bwr, bpdeg, bpscan;
   bool rlay = false;
   bool rec32 = false;  // 8-byte records (unit mode, every graph's 2m < 2^31)
   vid* c = nullptr; vid* p = nullptr; vid* nx = nullptr;
@@ -470,11 +467,21 @@ struct Engine {
     rlay = o.record_layout == MGL_LAYOUT_BINNED ||
            (o.record_layout == MGL_LAYOUT_AUTO && gmax > LAYOUT_AUTO_MIN_N);
     if (rlay) {
+      const int wb = (int)wsize(g.mode);
       HIPC(ctx, brpos.ensure(4 * (size_t)n0));
       HIPC(ctx, brinv.ensure(4 * (size_t)n0));
       HIPC(ctx, bcolr.ensure(4 * (size_t)std::max<int64_t>(nnz0, 1)));
+      HIPC(ctx, boffr.ensure(8 * ((size_t)n0 + 1)));
+      HIPC(ctx, bpdeg.ensure(8 * ((size_t)n0 + 1)));
+      HIPC(ctx, bpscan.ensure(DevScan::temp_bytes(n0 + 1)));
+      if (wb) HIPC(ctx, bwr.ensure((size_t)wb * (size_t)std::max<int64_t>(nnz0, 1)));
       launch_record_layout(bperm.as<vid>(), g.n, brpos.as<vid>(), brinv.as<vid>(), st());
-      launch_remap_cols(g.col.as<vid>(), g.nnz, brpos.as<vid>(), bcolr.as<vid>(), st());
+      launch_pos_degrees(g.off.as<eid>(), brinv.as<vid>(), g.n, bpdeg.as<int64_t>(), st());
+      scan_exclusive_i64(bpdeg.as<int64_t>(), boffr.as<int64_t>(), g.n + 1, bpscan.p, st());
+      for (int b = 1; b < nb; ++b)
+        launch_place_rows(L.b[b].upper, g.off.as<eid>(), g.col.as<vid>(), g.w.p, wb,
+                          brinv.as<vid>(), brpos.as<vid>(), boffr.as<eid>(), bin_off[b], bin_n[b],
+                          bcolr.as<vid>(), wb ? bwr.p : nullptr, st());
     }
     // hub plan of the last bin: chunks of each row, hash buckets per hub
     hub = HubPlan();
@@ -484,10 +491,22 @@ struct Engine {
       const vid* hperm = bperm.as<vid>() + bin_off[last];
       DBuf rr;
       HIPC(ctx, rr.ensure(16 * m));
-      launch_row_ranges(g.off.as<eid>(), hperm, m, rr.as<int64_t>(), st());
       std::vector<int64_t> ranges(2 * m);
       std::vector<vid> hubv(m);
-      HIPC(ctx, hipMemcpyAsync(ranges.data(), rr.p, 16 * m, hipMemcpyDeviceToHost, st()));
+      if (rlay) {
+        // the hub rows as placed: positions bin_off[last] .. bin_off[last] + m
+        std::vector<int64_t> ho(m + 1);
+        HIPC(ctx, hipMemcpyAsync(ho.data(), boffr.as<int64_t>() + bin_off[last], 8 * (m + 1),
+                                 hipMemcpyDeviceToHost, st()));
+        HIPC(ctx, hipStreamSynchronize(st()));
+        for (int64_t i = 0; i < m; ++i) {
+          ranges[2 * i] = ho[i];
+          ranges[2 * i + 1] = ho[i + 1];
+        }
+      } else {
+        launch_row_ranges(g.off.as<eid>(), hperm, m, rr.as<int64_t>(), st());
+        HIPC(ctx, hipMemcpyAsync(ranges.data(), rr.p, 16 * m, hipMemcpyDeviceToHost, st()));
+      }
       HIPC(ctx, hipMemcpyAsync(hubv.data(), hperm, 4 * m, hipMemcpyDeviceToHost, st()));
       HIPC(ctx, hipStreamSynchronize(st()));
       const int64_t CH = hub_chunk_edges();
@@ -622,9 +641,9 @@ struct Engine {
   MoveArgs move_args(const vid* from, vid* to, int up, void* vin) {
     LevelGraph& g = LG();
     MoveArgs A;
-    A.off = g.off.as<eid>();
+    A.off = rlay ? boffr.as<eid>() : g.off.as<eid>();
     A.col = rlay ? bcolr.as<vid>() : g.col.as<vid>();
-    A.w = g.w.p;
+    A.w = (rlay && wsize(g.mode)) ? bwr.p : g.w.p;
     A.k = bk.p;
     A.c = from;
     A.a = ba.p;
@@ -1229,9 +1248,11 @@ mgl_status Engine::level_end(int level) {
   const int nxt = (cur == 1) ? 2 : 1;
   LevelGraph& ng = lg[nxt];
   ng.mode = coarse_mode(g.mode, rec32);
-  // the columns as record positions belong to this level's move kernels; the
-  // next level makes its own (make_bins), so they leave room for the coarsening
+  // the placed rows belong to this level's move kernels; the next level makes
+  // its own (make_bins), so they leave room for the coarsening
   bcolr.release();
+  bwr.release();
+  bpdeg.release();
   // upper bounds: C <= n, E2 <= e_keep
   HIPC(ctx, ng.off.ensure(8 * (g.n + 2)));
   HIPC(ctx, ng.col.ensure(4 * std::max<int64_t>(e_keep, 1)));
@@ -2470,17 +2491,12 @@ static mgl_status load_tuning(mgl_ctx* ctx, const char* path, Tuning& tu) {
         FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d threads %d", path, i, b.s);
       if (b.size < 1 || (tu.table_check && b.upper > b.size))
         FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table too small", path, i);
-    } else if (b.shape == SHAPE_PHASE) {
-      if (b.s != 512 && b.s != 1024)
-        FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d threads %d", path, i, b.s);
-      if (b.size < 1) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d table size", path, i);
     } else if (b.shape == SHAPE_LANE) {
       if (b.upper > 16) FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: lane bin above 16", path);
     } else if (b.shape != SHAPE_GLOBAL) {
       FAIL(ctx, MGL_INVALID_ARG, "bad tuning file %s: bin %d shape", path, i);
     }
-    if (b.shape == SHAPE_GROUP || b.shape == SHAPE_BLOCK || b.shape == SHAPE_LANE ||
-        b.shape == SHAPE_PHASE) {
+    if (b.shape == SHAPE_GROUP || b.shape == SHAPE_BLOCK || b.shape == SHAPE_LANE) {
       // every mode instantiates the same shapes
       char kn[128];
       if (b.upper == INT64_MAX || move_kernel_name(MODE_UNIT, 1, b, kn, (int)sizeof(kn)) != 0)

@@ -11,16 +11,12 @@
 //   SHAPE_GLOBAL: hub path (mgl_hub.hip): rows split into chunks, candidates
 //                 bucketed by hash, one workgroup per bucket
 //   SHAPE_LANE  : one lane per vertex, the row merged in registers (degree <= 16)
-//   SHAPE_PHASE : one workgroup of BS threads per vertex, the row's records kept
-//                 in registers and aggregated in phases (hash ranges) in an LDS
-//                 hash of `size` slots, which may be smaller than the row
-enum Shape { SHAPE_NONE = 0, SHAPE_GROUP = 1, SHAPE_BLOCK = 2, SHAPE_GLOBAL = 3, SHAPE_LANE = 4,
-             SHAPE_PHASE = 5 };
+enum Shape { SHAPE_NONE = 0, SHAPE_GROUP = 1, SHAPE_BLOCK = 2, SHAPE_GLOBAL = 3, SHAPE_LANE = 4 };
 struct BinSpec {
   int64_t upper;  // inclusive degree bound (INT64_MAX for the last bin)
   int shape;
-  int s;          // lanes per vertex (GROUP) or threads per block (BLOCK, PHASE)
-  int size;       // GROUP: log2 of slots per vertex; BLOCK, PHASE: slots per vertex
+  int s;          // lanes per vertex (GROUP) or threads per block (BLOCK)
+  int size;       // GROUP: log2 of slots per vertex; BLOCK: slots per vertex
 };
 #define MGL_MAX_BINS 16
 
@@ -34,9 +30,10 @@ BinLayout default_bins(int lds_bytes, int slot_bytes);
 
 // ---- move ------------------------------------------------------------------
 struct MoveArgs {
-  const eid* off;
+  const eid* off;     // row offsets by record position (the row of v at off[rpos[v]]; by
+                      // vertex id when rpos is null)
   const vid* col;     // columns as record positions (rpos[u] when rpos is set)
-  const void* w;
+  const void* w;      // weights of those entries
   const void* k;      // K[n]
   const vid* c;       // assignment the moves start from
   const void* a;      // K[n] community totals of c
@@ -88,7 +85,16 @@ void launch_build_rec(int mode, int rec32, int64_t n, const vid* c, const void* 
 // Record layout of a level: rpos[perm[i]] = i and rinv[i] = perm[i] for i < n;
 // colr[e] = rpos[col[e]] for e < nnz.
 void launch_record_layout(const vid* perm, int64_t n, vid* rpos, vid* rinv, hipStream_t st);
-void launch_remap_cols(const vid* col, int64_t nnz, const vid* rpos, vid* colr, hipStream_t st);
+// Rows of a level in record-position order (records in degree-bin order):
+// deg[0..n] (degree of the vertex at each position, then 0), whose exclusive
+// scan gives offr; then the rows of positions p0 .. p0+np-1 (one bin; upper =
+// its degree bound picks the lanes per row) are placed at offr with columns
+// as record positions and their weights (wb bytes each, 0: none).
+void launch_pos_degrees(const eid* off, const vid* rinv, int64_t n, int64_t* deg,
+                        hipStream_t st);
+void launch_place_rows(int64_t upper, const eid* off, const vid* col, const void* w, int wb,
+                       const vid* rinv, const vid* rpos, const eid* offr, int64_t p0, int64_t np,
+                       vid* colr, void* wr, hipStream_t st);
 void move_kernels_allow_lds(int bytes);
 
 // ---- hub path (mgl_hub.hip) ------------------------------------------------

@@ -138,12 +138,50 @@ __global__ __launch_bounds__(256) void k_record_layout(const vid* __restrict__ p
   }
 }
 
-__global__ __launch_bounds__(256) void k_remap_cols(const vid* __restrict__ col, int64_t nnz,
+// Rows in record-position order: deg[p] = degree of the vertex at position p
+// (rinv[p]), deg[n] = 0; its exclusive scan is the offsets of the placed rows.
+__global__ __launch_bounds__(256) void k_pos_degrees(const eid* __restrict__ off,
+                                                     const vid* __restrict__ rinv, int64_t n,
+                                                     int64_t* __restrict__ deg) {
+  for (int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x; p <= n;
+       p += (int64_t)gridDim.x * 256) {
+    if (p == n) {
+      deg[p] = 0;
+    } else {
+      const vid v = rinv[p];
+      deg[p] = off[v + 1] - off[v];
+    }
+  }
+}
+
+// Places the rows of positions p0 .. p0+np-1: the row of v = rinv[p] goes to
+// offr[p] .. offr[p+1], its columns as record positions (rpos), its weights
+// (wb bytes each, 0 for unit weights) unchanged. S lanes per row, a workgroup
+// per row when S = 256.
+template <int S>
+__global__ __launch_bounds__(256) void k_place_rows(const eid* __restrict__ off,
+                                                    const vid* __restrict__ col,
+                                                    const void* __restrict__ w, int wb,
+                                                    const vid* __restrict__ rinv,
                                                     const vid* __restrict__ rpos,
-                                                    vid* __restrict__ colr) {
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nnz;
-       e += (int64_t)gridDim.x * 256)
-    colr[e] = rpos[colload(col, e)];
+                                                    const eid* __restrict__ offr, int64_t p0,
+                                                    int64_t np, vid* __restrict__ colr,
+                                                    void* __restrict__ wr) {
+  constexpr int GPB = S >= 256 ? 1 : 256 / S;
+  constexpr int STEP = S >= 256 ? 256 : S;
+  const int grp = S >= 256 ? 0 : (int)threadIdx.x / S;
+  const int lane = S >= 256 ? (int)threadIdx.x : (int)(threadIdx.x & (S - 1));
+  for (int64_t i = (int64_t)blockIdx.x * GPB + grp; i < np; i += (int64_t)gridDim.x * GPB) {
+    const int64_t p = p0 + i;
+    const vid v = rinv[p];
+    const eid s0 = off[v], s1 = off[v + 1], d0 = offr[p];
+    for (eid e = s0 + lane; e < s1; e += STEP) {
+      const eid q = d0 + (e - s0);
+      colr[q] = rpos[colload(col, e)];
+      if (wb == 4) ((uint32_t*)wr)[q] = ((const uint32_t*)w)[e];
+      else if (wb == 8) ((uint64_t*)wr)[q] = ((const uint64_t*)w)[e];
+    }
+  }
 }
 
 void launch_record_layout(const vid* perm, int64_t n, vid* rpos, vid* rinv, hipStream_t st) {
@@ -151,9 +189,27 @@ void launch_record_layout(const vid* perm, int64_t n, vid* rpos, vid* rinv, hipS
   k_record_layout<<<grid_for(n, 256, 16384), 256, 0, st>>>(perm, n, rpos, rinv);
 }
 
-void launch_remap_cols(const vid* col, int64_t nnz, const vid* rpos, vid* colr, hipStream_t st) {
-  if (nnz <= 0) return;
-  k_remap_cols<<<grid_for(nnz, 256, 65536), 256, 0, st>>>(col, nnz, rpos, colr);
+void launch_pos_degrees(const eid* off, const vid* rinv, int64_t n, int64_t* deg,
+                        hipStream_t st) {
+  k_pos_degrees<<<grid_for(n + 1, 256, 16384), 256, 0, st>>>(off, rinv, n, deg);
+}
+
+void launch_place_rows(int64_t upper, const eid* off, const vid* col, const void* w, int wb,
+                       const vid* rinv, const vid* rpos, const eid* offr, int64_t p0, int64_t np,
+                       vid* colr, void* wr, hipStream_t st) {
+  if (np <= 0) return;
+#define P_(S_)                                                                                 \
+  {                                                                                            \
+    constexpr int GPB = S_ >= 256 ? 1 : 256 / S_;                                              \
+    k_place_rows<S_><<<grid_for(np, GPB, 65536), 256, 0, st>>>(off, col, w, wb, rinv, rpos, \
+                                                               offr, p0, np, colr, wr);     \
+    return;                                                                                    \
+  }
+  if (upper <= 8) P_(4)
+  if (upper <= 64) P_(16)
+  if (upper <= 2048) P_(64)
+  P_(256)
+#undef P_
 }
 
 // ---------------------------------------------------------------------------
@@ -181,7 +237,7 @@ BinLayout default_bins(int lds_bytes, int slot_bytes) {
   const int64_t half = (int64_t)(lds_bytes / 2 - 1024) / slot_bytes;
   if (up > 8192 && half * 7 >= 8192 * 8) L.b[i++] = {8192, SHAPE_BLOCK, 512, (int)half};
   // above it one 1024-thread workgroup per CU with the whole LDS: on R-MAT 24
-  // the phased kernel (two 512-thread workgroups per CU, half tables, two
+  // a phased kernel (two 512-thread workgroups per CU, half tables, two
   // phases) took 2.36 ms for these rows against 1.86 ms
   if (up > 4096) L.b[i++] = {up, SHAPE_BLOCK, 1024, (int)hmax};
   L.b[i++] = {INT64_MAX, SHAPE_GLOBAL, 0, 0};

@@ -144,7 +144,7 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
     const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
     ci = rv.c;
     acik = (KT)rv.a;
-    scan_row<M, AT, true>(A, vp, ci, A.off[v], A.off[v + 1], lane, S, K, V, Av, LG, eici, selfw,
+    scan_row<M, AT, true>(A, vp, ci, A.off[vp], A.off[vp + 1], lane, S, K, V, Av, LG, eici, selfw,
                           pc);
   }
   __syncthreads();
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   __syncthreads();
   ST eici = 0, selfw = 0;
   ProbeCount pc;
-  scan_row<M, AT, false>(A, vp, ci, A.off[v], A.off[v + 1], threadIdx.x, BS, K, V, Av, H, eici,
+  scan_row<M, AT, false>(A, vp, ci, A.off[vp], A.off[vp + 1], threadIdx.x, BS, K, V, Av, H, eici,
                          selfw, pc);
   if constexpr (CNT) flush_probes(A.probes, pc);
   flag_overflow(A.err, pc);
@@ -333,8 +333,8 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
     const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
     ci = rv.c;
     acik = (KT)rv.a;
-    e0 = A.off[v];
-    e1 = A.off[v + 1];
+    e0 = A.off[vp];
+    e1 = A.off[vp + 1];
   }
   own_gather<AT, IT>(A, e0, e1, lane, S, u, r);  // no entries unless ok
   lds_barrier();  // the tables are cleared
@@ -395,7 +395,7 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   const vid vp = rec_pos(A, v, gi);
   const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
   const vid ci = rv.c;
-  const eid e0 = A.off[v], e1 = A.off[v + 1];
+  const eid e0 = A.off[vp], e1 = A.off[vp + 1];
   ST eici = 0, selfw = 0;
   int own[IT];
   AT oa[IT];
@@ -434,192 +434,6 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
 }
 
 // ---------------------------------------------------------------------------
-// SHAPE_PHASE: one workgroup per vertex with a table smaller than the row.
-// The row's records are gathered once and stay in registers; its candidates
-// are then aggregated in phases, each phase taking the communities whose
-// phase hash lies in one range of the 32-bit hash space, in a table of H
-// slots that is cleared between phases. With tables of half the workgroup
-// LDS, two workgroups share a CU, so one gathers while the other aggregates.
-// Phase ranges start as 2^k equal parts, k chosen so that the row's candidate
-// entries (an upper bound of its distinct communities) fill a table to at
-// most 4/5 on average. A phase whose communities overfill the table is
-// discarded (its argmax offers are not made) and its range is split in two;
-// phase_hash is a bijection on 32 bits, so a range of one value holds one
-// community and the splitting ends. Every candidate is offered exactly once
-// to the same total-order argmax, so the result equals the one-table kernels'.
-__device__ __forceinline__ uint32_t phase_hash(vid key) {
-  uint32_t x = (uint32_t)key * 0x2C1B3C6Du;  // odd multiplier: bijective
-  x ^= x >> 12;                              // xorshift: bijective
-  x *= 0x297A2D39u;
-  x ^= x >> 15;
-  return x;
-}
-#define PH_STACK 64
-
-// Slot of a key this thread inserted (it is in the table).
-template <class TB>
-__device__ __forceinline__ uint32_t tab_find(const TB& tb, vid key, uint32_t H) {
-  uint32_t h = any_slot(key, H);
-  const uint32_t st = probe_step(key, H, false, 0);
-  while (tb.key((int)h) != key) h = probe_next(h, st, H);
-  return h;
-}
-
-// (at least four wavefronts per SIMD: two 512-thread workgroups per CU, so at
-// most 128 VGPRs)
-template <class M, class AT, int BS, int IT, bool CNT>
-__global__ __launch_bounds__(BS, 4) void k_move_block_ph(MoveArgs A, const vid* __restrict__ perm,
-                                                      int64_t nb, uint32_t H) {
-  typedef typename M::S ST;
-  typedef typename M::K KT;
-  constexpr bool PACK = sizeof(ST) == 4;
-  typedef OwnTab<ST, PACK> TB;
-  static_assert(IT <= 32, "phase membership is a 32-bit mask per thread");
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double rg[BS / 64];
-  __shared__ vid rd[BS / 64];
-  __shared__ ST rs[BS / 64];
-  __shared__ int rc[BS / 64];
-  __shared__ uint32_t qlo[PH_STACK];  // pending ranges: [qlo, qlo + 2^(32 - qlg))
-  __shared__ int qlg[PH_STACK];
-  __shared__ int qn, ovf;
-  const int64_t gi = blockIdx.x;
-  if (gi >= nb) return;
-  const vid v = perm[gi];
-  int g;
-  if (!vertex_active<M>(A, v, g)) return;  // uniform over the workgroup
-  TB tb;
-  tb.init(smem, H);
-  for (uint32_t i = threadIdx.x; i < H; i += BS) tb.clear(i);
-  const vid vp = rec_pos(A, v, gi);
-  const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
-  const vid ci = rv.c;
-  const eid e0 = A.off[v], e1 = A.off[v + 1];
-  // gather the row; the columns are not kept (the self-loop entries are
-  // remembered as a bit mask), which leaves the registers to the records
-  Rec<AT> r[IT];
-  uint32_t selfm = 0;
-  {
-    const Rec<AT>* R = (const Rec<AT>*)A.rec;
-    vid u[IT];
-#pragma unroll
-    for (int j = 0; j < IT; ++j) {
-      const eid e = e0 + threadIdx.x + (eid)j * BS;
-      u[j] = e < e1 ? colload(A.col, e) : MGL_EMPTY;
-    }
-#pragma unroll
-    for (int j = 0; j < IT; ++j) {
-      selfm |= (uint32_t)(u[j] == vp) << j;
-      if (u[j] != MGL_EMPTY) r[j] = R[u[j]];
-      else r[j].c = MGL_EMPTY;
-    }
-  }
-  // keep the candidates (r[j].c, else EMPTY), sum the own community's entries
-  const bool v1 = A.version == 1;
-  ST eici = 0, selfw = 0;
-  ST wv[M::HAS_W ? IT : 1];
-  int ncand = 0;
-#pragma unroll
-  for (int j = 0; j < IT; ++j) {
-    const vid cj = r[j].c;
-    if (cj == MGL_EMPTY) continue;
-    const ST w = wload<M>(A.w, e0 + threadIdx.x + (eid)j * BS);
-    if constexpr (M::HAS_W) wv[j] = w;
-    const bool cand = v1 ? (A.up ? cj < ci : cj > ci) : (cj != ci);
-    if (cand) {
-      ++ncand;
-    } else {
-      if (cj == ci) {
-        if (!((selfm >> j) & 1u)) eici += w; else selfw += w;
-      }
-      r[j].c = MGL_EMPTY;
-    }
-  }
-  eici = block_sum<BS>(eici, rs);  // also orders the table clear before the inserts
-  selfw = block_sum<BS>(selfw, rs);
-  ncand = block_sum<BS>(ncand, rc);
-  if (threadIdx.x == 0) {
-    const uint32_t cap = H / 5 * 4 > 0 ? H / 5 * 4 : 1;
-    int lg = 0;
-    while (lg < 5 && ((uint32_t)ncand + (1u << lg) - 1) >> lg > cap) ++lg;
-    for (int p = 0; p < (1 << lg); ++p) {
-      qlo[p] = lg ? (uint32_t)p << (32 - lg) : 0u;
-      qlg[p] = lg;
-    }
-    qn = 1 << lg;
-    ovf = 0;
-  }
-  const KT ki = ((const KT*)A.k)[v];
-  const KT aci_k = (KT)rv.a - ki;
-  const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
-  const bool small = A.tie_small ? true : (A.up != 0);
-  Argmax<M, AT> am(ci);
-  int cnt = 0;
-  ProbeCount pc;
-  while (true) {
-    __syncthreads();  // the stack, the cleared table and ovf are visible
-    const int n = qn;
-    if (n == 0) break;
-    const uint32_t lo = qlo[n - 1];
-    const int lg = qlg[n - 1];
-    uint32_t mine = 0;
-    // the records are opaque to the compiler here, so that nothing computed
-    // from them (hashes, probe starts, conversions) is hoisted out of the
-    // phase loop and kept live in registers across it
-#pragma unroll
-    for (int j = 0; j < IT; ++j) asm volatile("" : "+v"(r[j].c), "+v"(r[j].a));
-#pragma unroll
-    for (int j = 0; j < IT; ++j) {
-      if (r[j].c == MGL_EMPTY) continue;
-      const uint32_t h = phase_hash(r[j].c);
-      if (lg && (h - lo) >> (32 - lg)) continue;  // outside [lo, lo + 2^(32-lg))
-      ST w = (ST)1;
-      if constexpr (M::HAS_W) w = wv[j];
-      if (tb.ins(r[j].c, w, H, false, 0, pc) >= 0) mine |= 1u << j;
-    }
-    if (pc.fail) {
-      ovf = 1;
-      pc.fail = 0;
-    }
-    __syncthreads();  // inserts done, ovf final, the range read by everyone
-    const bool failed = ovf != 0;
-    if (!failed) {
-#pragma unroll
-      for (int j = 0; j < IT; ++j) {
-        if (!((mine >> j) & 1u)) continue;
-        ++cnt;
-        const uint32_t s = tab_find(tb, r[j].c, H);
-        am.offer(G, tb.val((int)s), r[j].a, r[j].c, small);
-      }
-    }
-    __syncthreads();  // the table is read before it is cleared
-    if (threadIdx.x == 0) {
-      qn = n - 1;
-      if (failed) {  // lg < 32: a one-value range holds one community
-        qlo[n - 1] = lo;
-        qlg[n - 1] = lg + 1;
-        qlo[n] = lo + (1u << (31 - lg));
-        qlg[n] = lg + 1;
-        qn = n + 1;
-      }
-      ovf = 0;
-    }
-    if (n > 1 || failed)
-      for (uint32_t i = threadIdx.x; i < H; i += BS) tb.clear(i);
-  }
-  if constexpr (CNT) flush_probes(A.probes, pc);
-  double best = am.best;
-  vid dst = am.dst;
-  block_argmax<BS, false>(best, dst, small, rg, rd);  // rg, rd unused before
-  if (CNT && A.stats) cnt = block_sum<BS>(cnt, rc);
-  if (threadIdx.x == 0) {
-    A.c_new[v] = clamp_dst(dst, ci, A.up, A.version);
-    if (A.vin) ((KT*)A.vin)[v] = (KT)eici + (KT)selfw;
-    if (CNT && A.stats) atomicAdd(A.stats, (unsigned long long)cnt);
-  }
-}
-
-// ---------------------------------------------------------------------------
 // SHAPE_LANE: one lane per vertex for rows of at most D entries. The row is
 // gathered into registers and equal communities are merged by pairwise
 // comparison (the first occurrence accumulates), so no LDS, barrier or
@@ -643,8 +457,8 @@ __global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __rest
     const vid vp = rec_pos(A, v, gi);
     const Rec<AT> rv = R[vp];
     const vid ci = rv.c;
-    const eid e0 = A.off[v];
-    const int d = (int)(A.off[v + 1] - e0);
+    const eid e0 = A.off[vp];
+    const int d = (int)(A.off[vp + 1] - e0);
     vid u[D];
     Rec<AT> r[D];
     ST w[D];
@@ -797,19 +611,6 @@ static int move_bin_t(const BinSpec& bs, const MoveArgs& A, const vid* perm, int
       k_move_block<M, AT, 1024, CNT><<<(unsigned)nb, 1024, lds, st>>>(A, perm, nb, H);
       return 0;
     }
-  } else if (bs.shape == SHAPE_PHASE) {
-    const uint32_t H = (uint32_t)bs.size;
-    const size_t lds_own = (sizeof(ST) == 4) ? 8 * (size_t)H
-                                             : ((4 * (size_t)H + 15) / 16) * 16 + sizeof(ST) * (size_t)H;
-    const int64_t it = (bs.upper + bs.s - 1) / bs.s;
-#define P_(BS_, IT_)                                                                        \
-  if (bs.s == BS_ && it <= IT_) {                                                           \
-    NAME_("k_move_block_ph<%s, %s, %d, %d, %s>", mn, an, BS_, IT_, cn)                              \
-    k_move_block_ph<M, AT, BS_, IT_, CNT><<<(unsigned)nb, BS_, lds_own, st>>>(A, perm, nb, H);   \
-    return 0;                                                                               \
-  }
-    P_(512, 16) P_(512, 32) P_(1024, 16)
-#undef P_
   }
 #undef NAME_
   return -1;
@@ -828,11 +629,6 @@ static void move_allow_lds_cnt(int bytes) {
 #define P_(BS_) A_(BS_) O_(BS_, 1) O_(BS_, 2) O_(BS_, 4) O_(BS_, 8) O_(BS_, 16)
   P_(256) P_(512) P_(1024)
 #undef P_
-#define H_(BS_, IT_)                                                                 \
-  (void)hipFuncSetAttribute((const void*)k_move_block_ph<M, AT, BS_, IT_, CNT>,      \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-  H_(512, 16) H_(512, 32) H_(1024, 16)
-#undef H_
 #undef O_
 #undef A_
 }

@@ -301,9 +301,7 @@ _GOOD_BINS = "0 0 0 0\n32 1 32 6\n-1 3 0 0\n"
     ("0 0 0 0\n16 1 512 5\n-1 3 0 0\n", "lanes per vertex"),
     ("0 0 0 0\n16 1 4 6\n-1 3 0 0\n", "no compiled kernel"),
     ("0 0 0 0\n512 2 128 600\n-1 3 0 0\n", "threads"),
-    ("0 0 0 0\n16384 5 256 1021\n-1 3 0 0\n", "threads"),
-    ("0 0 0 0\n16384 5 512 0\n-1 3 0 0\n", "table size"),
-    ("0 0 0 0\n32768 5 512 1021\n-1 3 0 0\n", "no compiled kernel"),
+    ("0 0 0 0\n16384 5 512 1021\n-1 3 0 0\n", "shape"),
 ])
 def test_tuning_file_errors(ctx, mgl, tmp_path, text, reason):
     # every malformed tuning file is refused with INVALID_ARG naming the
@@ -369,50 +367,6 @@ def test_hub_stream_placement_parity(ctx, mgl, orc, tmp_path, tuning):
         np.testing.assert_array_equal(
             s.move(up), orc.move(g.offsets, g.cols, None, k, c, a, g.nnz, up))
     s.close()
-
-
-# The phased kernel (shape 5) forced with tables far smaller than the rows:
-# rows of 65-512 entries in 61 slots (up to 32 phases of about 16
-# candidates), rows of 513-2048 in 1024-thread workgroups with 127 slots, rows
-# of 2049-16384 in 61 slots (32 phases of 64-512 candidates: a phase overfills
-# the table and is split, repeatedly, until its communities fit).
-PHASE_TUNE_BINS = "".join(
-    f"{u} {sh} {s_} {sz}\n" for u, sh, s_, sz in
-    [(0, 0, 0, 0), (4, 1, 4, 3), (8, 1, 8, 4), (16, 1, 16, 5), (64, 1, 64, 7),
-     (512, 5, 512, 61), (2048, 5, 1024, 127), (16384, 5, 512, 61), (-1, 3, 0, 0)])
-
-
-@pytest.mark.parametrize("version", [0, 1, 2])
-@pytest.mark.parametrize("form", ["unit", "exact", "fp64"])
-def test_phased_kernel_random_assignments(ctx, mgl, orc, tmp_path, form, version):
-    g = hub_graph(mgl, weights=0 if form == "unit" else 1)
-    tf = tmp_path / "tune.txt"
-    tf.write_text(PHASE_TUNE_BINS)
-    prec = {"unit": mgl.PREC_EXACT, "exact": mgl.PREC_EXACT, "fp64": mgl.PREC_FP64}[form]
-    s = ctx.session(g, tuning_file=str(tf), version=version, precision=prec)
-    bins = s.bins()
-    print(f"{form} v{version} bins {bins}", flush=True)
-    assert len(bins) == 9 and all(b > 0 for b in bins[5:8])
-    ho = HubOracle(orc, g, form)
-    rng = np.random.default_rng(17)
-    for trial in range(3):
-        c = rng.integers(0, g.n, g.n).astype(np.int32)
-        if trial > 0:
-            c = (c // (9 * trial)) * (9 * trial)
-        s.set_assign(c)
-        for up in (True, False):
-            np.testing.assert_array_equal(s.move(up), ho.move(c, up, version))
-    s.close()
-
-
-@pytest.mark.parametrize("version", [0, 1, 2])
-def test_phased_kernel_full_parity(ctx, mgl, orc, tmp_path, version):
-    tf = tmp_path / "tune.txt"
-    tf.write_text(PHASE_TUNE_BINS)
-    g = hub_graph(mgl)
-    res = ctx.louvain(g, tuning_file=str(tf), version=version, keep_levels=True,
-                      keep_graphs=True)
-    compare(res, orc.louvain(g.offsets, g.cols, None, version=version))
 
 
 def test_edge_cases(ctx, mgl, orc):
