@@ -286,7 +286,7 @@ struct Engine {
   // order (offr: offsets by position, colr: columns as record positions, wr:
   // their weights), so each bin's rows are contiguous; pdeg and pscan are
   // the scratch of offr's scan
-  DBuf brpos, brinv, bcolr, boffr, bwr, bpdeg, bpscan;
+  DBuf brpos, brinv, bcolr, boffr, bwr, bpdeg, bpscan, bkr;  // bkr: k by position
   bool rlay = false;
   bool rec32 = false;  // 8-byte records (unit mode, every graph's 2m < 2^31)
   vid* c = nullptr; vid* p = nullptr; vid* nx = nullptr;
@@ -644,7 +644,7 @@ struct Engine {
     A.off = rlay ? boffr.as<eid>() : g.off.as<eid>();
     A.col = rlay ? bcolr.as<vid>() : g.col.as<vid>();
     A.w = (rlay && wsize(g.mode)) ? bwr.p : g.w.p;
-    A.k = bk.p;
+    A.k = rlay ? bkr.p : bk.p;
     A.c = from;
     A.a = ba.p;
     A.rec = brec.p;
@@ -776,6 +776,11 @@ struct Engine {
     launch_fill_bytes0(bvin.p, 8 * (size_t)std::max<int64_t>(g.n, 1), st());
     s = rows(OP_WEIGHT, nullptr, bk.p, bvin.p, false);
     if (s) return s;
+    if (rlay) {
+      // the move kernels read k by record position, like the row offsets
+      HIPC(ctx, bkr.ensure(8 * (size_t)std::max<int64_t>(n0, 1)));
+      launch_gather_u64(bk.p, brinv.as<vid>(), g.n, bkr.p, st());
+    }
     launch_copy_k_to_a(g.mode, g.n, bk.p, ba.p, st());
     launch_iota(c, g.n, st());
     launch_iota(p, g.n, st());

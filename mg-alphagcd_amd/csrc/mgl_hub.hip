@@ -147,7 +147,7 @@ __global__ void k_hub_head(MoveArgs A, HubPlan H) {
     ((int64_t*)H.hub_self)[i] = 0;
     const vid v = H.hub_v[i];
     const vid ci = A.c[v];
-    const KT ki = ((const KT*)A.k)[v];
+    const KT ki = ((const KT*)A.k)[A.rpos ? A.rpos[v] : v];  // k by record position
     const KT acik = ((const KT*)A.a)[ci] - ki;
     HubGain hg;
     __builtin_memcpy(&hg.ki, &ki, 8);

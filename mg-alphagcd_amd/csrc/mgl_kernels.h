@@ -34,7 +34,7 @@ struct MoveArgs {
                       // vertex id when rpos is null)
   const vid* col;     // columns as record positions (rpos[u] when rpos is set)
   const void* w;      // weights of those entries
-  const void* k;      // K[n]
+  const void* k;      // K[n] vertex totals, by record position like off
   const vid* c;       // assignment the moves start from
   const void* a;      // K[n] community totals of c
   const void* rec;    // Rec<AT>[n]: {c[u], a[c[u]]} of the assignment the moves start from,
@@ -177,6 +177,8 @@ int hub_agg_cap();  // candidates of one bucket on the aggregate's owner path
 void launch_row_ranges(const eid* off, const vid* perm, int64_t nb, int64_t* out, hipStream_t st);
 // out[i] = src[idx[i]] for i < n
 void launch_gather_i32(const int32_t* src, const vid* idx, int64_t n, int32_t* out, hipStream_t st);
+// out[i] = src[idx[i]] for 8-byte values
+void launch_gather_u64(const void* src, const vid* idx, int64_t n, void* out, hipStream_t st);
 
 // ---- per-vertex row reductions (vertex weights, modularity sum_in) ----------
 struct RowArgs {

@@ -456,6 +456,17 @@ void launch_gather_i32(const int32_t* src, const vid* idx, int64_t n, int32_t* o
   k_gather_i32<<<grid_for(n, 256, 4096), 256, 0, st>>>(src, idx, n, out);
 }
 
+__global__ void k_gather_u64(const uint64_t* src, const vid* idx, int64_t n, uint64_t* out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = src[idx[i]];
+}
+void launch_gather_u64(const void* src, const vid* idx, int64_t n, void* out, hipStream_t st) {
+  if (n <= 0) return;
+  k_gather_u64<<<grid_for(n, 256, 16384), 256, 0, st>>>((const uint64_t*)src, idx, n,
+                                                         (uint64_t*)out);
+}
+
 // ---------------------------------------------------------------------------
 // Community-weight update: a[c] -= k, a[c_new] += k for every moved vertex of
 // an active graph (louvain_baseline.cu:1339-1361 form; louvain.cu:1328-1336

@@ -136,11 +136,11 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
   if (ok) { v = perm[gi]; ok = vertex_active<M>(A, v, g); }
   __syncthreads();
   ST eici = 0, selfw = 0;
-  vid ci = 0;
+  vid ci = 0, vp = 0;
   KT acik = 0;
   ProbeCount pc;
   if (ok) {
-    const vid vp = rec_pos(A, v, gi);
+    vp = rec_pos(A, v, gi);
     const Rec<AT> rv = ((const Rec<AT>*)A.rec)[vp];
     ci = rv.c;
     acik = (KT)rv.a;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_move_group(MoveArgs A, const vid* __res
   if (ok) {
     eici = group_sum<S>(eici);
     selfw = group_sum<S>(selfw);
-    const KT ki = ((const KT*)A.k)[v];
+    const KT ki = ((const KT*)A.k)[vp];
     const KT aci_k = acik - ki;
     const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
     const bool small = A.tie_small ? true : (A.up != 0);
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(BS) void k_move_block(MoveArgs A, const vid* __rest
   __syncthreads();
   eici = block_sum<BS>(eici, rs);
   selfw = block_sum<BS>(selfw, rs);
-  const KT ki = ((const KT*)A.k)[v];
+  const KT ki = ((const KT*)A.k)[vp];
   const KT aci_k = (KT)rv.a - ki;
   const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
   const bool small = A.tie_small ? true : (A.up != 0);
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_move_group_own(MoveArgs A, const vid* _
   if (ok) {
     eici = group_sum<S>(eici);
     selfw = group_sum<S>(selfw);
-    const KT ki = ((const KT*)A.k)[v];
+    const KT ki = ((const KT*)A.k)[vp];
     const KT aci_k = acik - ki;
     const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
     const bool small = A.tie_small ? true : (A.up != 0);
@@ -416,7 +416,7 @@ __global__ __launch_bounds__(BS) void k_move_block_own(MoveArgs A, const vid* __
   block_sum2<BS>(eici, selfw, rs, rs2);  // eici and the self-loop, one barrier
   // (loaded here: a load issued before the barrier above would be waited for
   // there; on R-MAT 24 rows of 4,097-8,192 entries that cost 0.08 ms a pass)
-  const KT ki = ((const KT*)A.k)[v];
+  const KT ki = ((const KT*)A.k)[vp];
   const KT aci_k = (KT)rv.a - ki;
   const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
   const bool small = A.tie_small ? true : (A.up != 0);
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_move_lane(MoveArgs A, const vid* __rest
         }
       }
     }
-    const KT ki = ((const KT*)A.k)[v];
+    const KT ki = ((const KT*)A.k)[vp];
     const KT aci_k = (KT)rv.a - ki;
     const GainIn G = gain_in<M, AT>(A.gs[g].mass, A.gs[g].mass_i, ki, aci_k, eici);
     const bool small = A.tie_small ? true : (A.up != 0);
