@@ -489,8 +489,6 @@ struct Engine {
     if (L.b[last].shape == SHAPE_GLOBAL && bin_n[last] > 0) {
       const int64_t m = bin_n[last];
       const vid* hperm = bperm.as<vid>() + bin_off[last];
-      DBuf rr;
-      HIPC(ctx, rr.ensure(16 * m));
       std::vector<int64_t> ranges(2 * m);
       std::vector<vid> hubv(m);
       if (rlay) {
@@ -504,8 +502,11 @@ struct Engine {
           ranges[2 * i + 1] = ho[i + 1];
         }
       } else {
+        DBuf rr;
+        HIPC(ctx, rr.ensure(16 * m));
         launch_row_ranges(g.off.as<eid>(), hperm, m, rr.as<int64_t>(), st());
         HIPC(ctx, hipMemcpyAsync(ranges.data(), rr.p, 16 * m, hipMemcpyDeviceToHost, st()));
+        HIPC(ctx, hipStreamSynchronize(st()));  // before rr is released
       }
       HIPC(ctx, hipMemcpyAsync(hubv.data(), hperm, 4 * m, hipMemcpyDeviceToHost, st()));
       HIPC(ctx, hipStreamSynchronize(st()));

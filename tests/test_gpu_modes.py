@@ -29,9 +29,12 @@ def agreement(a, b):
     return sum(best.values()) / len(a)
 
 
-def test_fp64_mode_matches_oracle(ctx, mgl, orc):
+@pytest.mark.parametrize("layout", ["AUTO", "BINNED"])
+def test_fp64_mode_matches_oracle(ctx, mgl, orc, layout):
+    # BINNED: records, rows and their FP64 weights placed in degree-bin order
     g = mgl.gen_sbm(1 << 13, 32, min_size=64, seed=21, weights=1)
-    res = ctx.louvain(g, precision=mgl.PREC_FP64, keep_levels=True)
+    res = ctx.louvain(g, precision=mgl.PREC_FP64, keep_levels=True,
+                      record_layout=getattr(mgl, "LAYOUT_" + layout))
     assert res.precision == mgl.PREC_FP64
     ref = orc.louvain(g.offsets, g.cols, g.w.astype(np.float64), mode=1)
     assert len(res.levels) == len(ref.levels)
@@ -51,10 +54,14 @@ def test_final_q_matches_fp64_recompute(ctx, mgl, orc, prec):
     assert abs(res.final_q - q64) <= 1e-6 * abs(q64)
 
 
-def test_fp32_hash_vs_fp64_lognormal(ctx, mgl, orc):
-    # lognormal(0,1) weights: neither mode is exact; report the difference
+@pytest.mark.parametrize("layout", ["AUTO", "BINNED"])
+def test_fp32_hash_vs_fp64_lognormal(ctx, mgl, orc, layout):
+    # lognormal(0,1) weights: neither mode is exact; report the difference.
+    # BINNED: the FP32-hash run reads its rows and FP32 weights placed in
+    # degree-bin order
     g = mgl.gen_rmat(18, 16, seed=5, weights=2)
-    r32 = ctx.louvain(g, precision=mgl.PREC_FP32_HASH)
+    r32 = ctx.louvain(g, precision=mgl.PREC_FP32_HASH,
+                      record_layout=getattr(mgl, "LAYOUT_" + layout))
     r64 = ctx.louvain(g, precision=mgl.PREC_FP64)
     q32 = orc.modularity_fp64(g.offsets, g.cols, g.w, r32.final_assign)
     q64 = orc.modularity_fp64(g.offsets, g.cols, g.w, r64.final_assign)
