@@ -243,6 +243,39 @@ def test_hub_path_random_assignments(ctx, mgl, orc, tmp_path, form, bucket, segm
     s.close()
 
 
+@pytest.mark.parametrize("threads", [512, 1024])
+@pytest.mark.parametrize("form", ["unit", "exact"])
+def test_hub_count_workgroup_sizes(ctx, mgl, orc, tmp_path, threads, form):
+    # hub_count_threads: the count step's workgroups of 512 and 1024 threads
+    # (8,192 entries each, so 16 and 8 entries per thread) give the default's
+    # proposals; many hubs (rows above 1,024 entries)
+    g = hub_graph(mgl, weights=0 if form == "unit" else 1)
+    tf = tmp_path / "tune.txt"
+    tf.write_text(f"hub_count_threads {threads}\n" + "".join(
+        f"{u} {sh} {s_} {sz}\n" for u, sh, s_, sz in
+        [(0, 0, 0, 0), (4, 1, 4, 3), (8, 1, 8, 4), (16, 1, 16, 5), (32, 1, 32, 6),
+         (64, 1, 64, 7), (128, 1, 64, 8), (256, 1, 64, 9), (1024, 2, 256, 2048),
+         (-1, 3, 0, 0)]))
+    s = ctx.session(g, tuning_file=str(tf))
+    ho = HubOracle(orc, g, form)
+    rng = np.random.default_rng(31)
+    for trial in range(2):
+        c = rng.integers(0, g.n, g.n).astype(np.int32)
+        if trial > 0:
+            c = (c // 11) * 11
+        s.set_assign(c)
+        for up in (True, False):
+            np.testing.assert_array_equal(s.move(up), ho.move(c, up, 2))
+    s.close()
+    res = ctx.louvain(g, tuning_file=str(tf), keep_levels=True)
+    w = None if g.w is None else g.w.astype(np.float64)
+    ref = orc.louvain(g.offsets, g.cols, w, mode=0)
+    assert len(res.levels) == len(ref.levels)
+    for lg_, lr in zip(res.levels, ref.levels):
+        assert lg_.q == lr["q"] and lg_.iters == lr["iters"]
+    np.testing.assert_array_equal(res.final_assign, ref.final)
+
+
 @pytest.mark.parametrize("threads,per_cu", [(256, 2), (512, 4), (1024, 2), (1024, 1)])
 def test_hub_aggregate_workgroup_shapes(ctx, mgl, orc, tmp_path, threads, per_cu):
     g = hub_graph(mgl)
@@ -286,6 +319,7 @@ _GOOD_BINS = "0 0 0 0\n32 1 32 6\n-1 3 0 0\n"
 
 @pytest.mark.parametrize("text, reason", [
     ("hub_agg_threads 300\n" + _GOOD_BINS, "hub_agg_threads"),
+    ("hub_count_threads 128\n" + _GOOD_BINS, "hub_count_threads"),
     ("hub_agg_per_cu 9\n" + _GOOD_BINS, "hub_agg_per_cu"),
     ("hub_bucket 0\n" + _GOOD_BINS, "hub_bucket"),
     ("hub_fork_upper -1\n" + _GOOD_BINS, "hub_fork_upper"),
