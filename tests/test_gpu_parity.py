@@ -484,6 +484,29 @@ def test_binned_layout_move_stage(ctx, mgl, orc):
     s.close()
 
 
+@pytest.mark.parametrize("version", [0, 1, 2])
+@pytest.mark.parametrize("form", ["unit", "exact", "fp64"])
+def test_binned_layout_weighted_move_stage(ctx, mgl, orc, form, version):
+    # records and rows (with their int64 or FP64 weights) placed in degree-bin
+    # order on a graph below the automatic size: every bin shape, the hub path
+    # included, reads the placed rows; random assignments against the oracle
+    g = hub_graph(mgl, weights=0 if form == "unit" else 1)
+    prec = {"unit": mgl.PREC_EXACT, "exact": mgl.PREC_EXACT, "fp64": mgl.PREC_FP64}[form]
+    s = ctx.session(g, record_layout=mgl.LAYOUT_BINNED, version=version, precision=prec)
+    bins = s.bins()
+    assert bins[-1] > 0, bins  # hub rows present
+    ho = HubOracle(orc, g, form)
+    rng = np.random.default_rng(41)
+    for trial in range(3):
+        c = rng.integers(0, g.n, g.n).astype(np.int32)
+        if trial > 0:
+            c = (c // (5 * trial)) * (5 * trial)
+        s.set_assign(c)
+        for up in (True, False):
+            np.testing.assert_array_equal(s.move(up), ho.move(c, up, version))
+    s.close()
+
+
 def test_autotune_layout_parity(ctx, mgl, orc, tmp_path):
     # the autotuner times candidate bin layouts and hub-aggregate shapes on the
     # graph and writes the fastest as a tuning file; runs under that file match
