@@ -240,6 +240,12 @@ def test_lds_table_overflow_is_a_device_error(ctx, mgl, tmp_path):
         s.move(True)
     assert e.value.code == mgl.DEVICE_ERROR
     s.close()
+    # a timed pass reads the error word with the graph states
+    s = ctx.session(g, tuning_file=str(tf))
+    with pytest.raises(mgl.MglError) as e:
+        s.pass_()
+    assert e.value.code == mgl.DEVICE_ERROR and "overflow" in str(e.value)
+    s.close()
     # without the switch the same layout is refused up front
     tf.write_text("0 0 0 0\n64 1 8 4\n-1 3 0 0\n")
     with pytest.raises(mgl.MglError) as e:
